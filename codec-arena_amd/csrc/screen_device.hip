@@ -1,0 +1,914 @@
+// screen_device.hip -- gfx950 screening funnel: candidate synthesis, quadratic
+// scoring and exact top-K for the codec-arena `arena screen` call.
+//
+// Reference semantics (codec-arena snapshot):
+//   src/screen.zig:195-213  splitmix64 stream, unit = (out >> 40) * 2^-24
+//   src/screen.zig:225-255  worker: per candidate 2 parent draws, immigrant
+//                           test, 29 x (crossover bit, mutation unit) or
+//                           29 immigrant units, jump test (+ 2 jump draws)
+//   src/screen.zig:97-112   Model.score, strict f32, ordered lane sum
+//   src/screen.zig:151-193  per-stream bounded min-heap
+//   src/screen.zig:257-320  stream partition, merge, stable sort
+//
+// Device pipeline for one call (all streams at once):
+//   1. k_map      every 65536-draw block of every stream: immigrant/jump event
+//                 bits of every draw (LDS bitmaps), then the transfer map
+//                 phase -> (exit phase, candidates) for all 64 entry phases.
+//   2. k_reduce / k_down   compose the maps in groups of 64 up to one map per
+//                 stream, then push the true entry (phase, first candidate
+//                 index) back down to every block.
+//   3. k_score    per block (one wavefront): walk the true candidate chain with
+//                 ballot speculation, synthesize each candidate in registers,
+//                 score it and emit only candidates above the stream's running
+//                 bound.  Launched in geometrically growing block ranges; the
+//                 host replays the emitted candidates through the reference
+//                 heap between launches, so the bound is the heap root of the
+//                 prefix already processed and the final heaps are exact.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "funnel_internal.h"
+
+namespace csf {
+
+constexpr uint64_t GAMMA = 0x9E3779B97F4A7C15ull;
+constexpr uint64_t MIX1 = 0xBF58476D1CE4E5B9ull;
+constexpr uint64_t MIX2 = 0x94D049BB133111EBull;
+
+constexpr int BLOCK_LOG = 16;
+constexpr uint64_t BLOCK = 1ull << BLOCK_LOG;  // draws per block
+constexpr int SUBSEG = 4096;                   // draws per walked sub-segment
+constexpr int NSUB = (int)(BLOCK / SUBSEG);    // 16
+constexpr int BITWORDS = (int)(BLOCK / 64) + 1;  // + 64 draws of look-ahead
+
+struct StreamInfo {
+    uint64_t seed;       // seed ^ (gamma * (t + 1))
+    uint64_t my_lambda;  // candidates of this stream
+    uint64_t blk_first;  // index of the stream's first block in block arrays
+    uint64_t nblocks;    // blocks mapped for this stream
+};
+
+struct LaunchSeg {  // one stream's block range in a scoring launch
+    uint64_t flat_first;
+    uint64_t b0;
+    float bound;  // NaN: accept every candidate (heap not full yet)
+    uint32_t _pad;
+};
+
+struct DevModel {
+    float w0;
+    float lin[CS_NTOT];
+    float tri[136];  // lower triangle, row-major: (i, j<=i) at i*(i+1)/2 + j
+};
+typedef const __attribute__((address_space(4))) DevModel *ConstModel;
+
+// ------------------------------------------------------------ splitmix64
+
+// state after n increments is seed + n*gamma; draw n is mix(that).  The
+// mixer is written on 32-bit halves so that each 64-bit shift is one
+// alignbit + one shift and each 64x64 product keeps only the words needed.
+struct Mid {
+    uint32_t lo, hi;  // u = x ^ (x >> 27), x = (s ^ (s >> 30)) * MIX1
+};
+__device__ __forceinline__ Mid mix_mid(uint64_t s) {
+    const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
+    const uint32_t tlo = lo ^ __builtin_amdgcn_alignbit(hi, lo, 30);
+    const uint32_t thi = hi ^ (hi >> 30);
+    const uint64_t p = (uint64_t)tlo * (uint32_t)MIX1;
+    const uint32_t xlo = (uint32_t)p;
+    const uint32_t xhi = (uint32_t)(p >> 32) + tlo * (uint32_t)(MIX1 >> 32) + thi * (uint32_t)MIX1;
+    return Mid{xlo ^ __builtin_amdgcn_alignbit(xhi, xlo, 27), xhi ^ (xhi >> 27)};
+}
+// high word of z = u * MIX2
+__device__ __forceinline__ uint32_t mix_zhi(Mid u) {
+    return __umulhi(u.lo, (uint32_t)MIX2) + u.lo * (uint32_t)(MIX2 >> 32) + u.hi * (uint32_t)MIX2;
+}
+// (out >> 32): parent / jump indices
+__device__ __forceinline__ uint32_t draw_hi(uint64_t s) {
+    const uint32_t h = mix_zhi(mix_mid(s));
+    return h ^ (h >> 31);
+}
+// (out >> 40): the 24-bit unit numerator
+__device__ __forceinline__ uint32_t draw_u24(uint64_t s) { return mix_zhi(mix_mid(s)) >> 8; }
+// (out & 1): crossover bit; needs only the low word of z
+__device__ __forceinline__ uint32_t draw_bit(uint64_t s) {
+    const uint32_t l = mix_mid(s).lo * (uint32_t)MIX2;
+    return (l ^ (l >> 31)) & 1u;
+}
+__device__ __forceinline__ float unit_f(uint32_t m) { return (float)m * 0x1p-24f; }
+
+// a % d for 32-bit a by Lemire, Kaser & Kurz, "Faster remainder by direct
+// computation" (2019): M = floor((2^64-1)/d) + 1.
+__device__ __forceinline__ uint32_t fastmod(uint32_t a, uint64_t M, uint32_t d) {
+    return (uint32_t)__umul64hi(M * (uint64_t)a, (uint64_t)d);
+}
+
+__device__ __forceinline__ int find_seg(const uint64_t *first, int T, uint64_t f) {
+    int lo = 0, hi = T;  // first[lo] <= f < first[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (first[mid] <= f) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t bit_at(const uint64_t *w, uint32_t p) {
+    return (uint32_t)(w[p >> 6] >> (p & 63)) & 1u;
+}
+
+// ---------------------------------------------------------------- k_map
+
+// One workgroup (4 waves) per block.  Bits: word i, bit l <-> draw
+// block_start + 64 i + l.  Candidate starts c (the draw counter before the
+// candidate) in [block_start, block_start + BLOCK) need the immigrant bit at
+// c+3 and the jump bit at c+33 (immigrant) or c+62.
+__global__ __launch_bounds__(256) void k_map(const StreamInfo *__restrict__ streams,
+                                             const uint64_t *__restrict__ blk_first, int T,
+                                             uint64_t nblk, uint32_t immT, uint32_t jumpT,
+                                             uint64_t *__restrict__ maps) {
+    __shared__ uint64_t imm_bits[BITWORDS];
+    __shared__ uint64_t jmp_bits[BITWORDS];
+    __shared__ uint16_t sub_map[NSUB][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (uint64_t f = blockIdx.x; f < nblk; f += gridDim.x) {
+        const int t = find_seg(blk_first, T, f);
+        const uint64_t b = f - blk_first[t];
+        const uint64_t seed = streams[t].seed;
+        const uint64_t p0 = b << BLOCK_LOG;
+        // 1. event bits: wave w covers words [256 w, 256 w + 256); wave 3 also word 1024
+        uint64_t s = seed + (p0 + (uint64_t)wave * 16384u + (uint64_t)lane) * GAMMA;
+        const uint64_t step = 64ull * GAMMA;
+#pragma unroll 4
+        for (int it = 0; it < 256; ++it) {
+            const uint32_t m = draw_u24(s);
+            const uint64_t bi = __ballot(m < immT);
+            const uint64_t bj = __ballot(m < jumpT);
+            if (lane == 0) {
+                imm_bits[wave * 256 + it] = bi;
+                jmp_bits[wave * 256 + it] = bj;
+            }
+            s += step;
+        }
+        if (wave == 3) {
+            const uint32_t m = draw_u24(s);
+            const uint64_t bi = __ballot(m < immT);
+            const uint64_t bj = __ballot(m < jumpT);
+            if (lane == 0) {
+                imm_bits[1024] = bi;
+                jmp_bits[1024] = bj;
+            }
+        }
+        __syncthreads();
+        // 2. walk: lane = entry phase, 4 interleaved sub-segments per wave
+        uint32_t c[4], e[4], n[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t base = (uint32_t)(wave * 4 + q) * SUBSEG;
+            c[q] = base + lane;
+            e[q] = base + SUBSEG;
+            n[q] = 0;
+        }
+        while (c[0] < e[0] || c[1] < e[1] || c[2] < e[2] || c[3] < e[3]) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (c[q] < e[q]) {
+                    const uint32_t im = bit_at(imm_bits, c[q] + 3);
+                    const uint32_t j33 = bit_at(jmp_bits, c[q] + 33);
+                    const uint32_t j62 = bit_at(jmp_bits, c[q] + 62);
+                    const uint32_t L = im ? (j33 ? 35u : 33u) : (j62 ? 64u : 62u);
+                    c[q] += L;
+                    n[q] += 1;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sub_map[wave * 4 + q][lane] = (uint16_t)((n[q] << 6) | (c[q] - e[q]));
+        __syncthreads();
+        // 3. compose the 16 sub-maps into the block map
+        if (wave == 0) {
+            uint32_t ph = (uint32_t)lane;
+            uint64_t cnt = 0;
+#pragma unroll
+            for (int k = 0; k < NSUB; ++k) {
+                const uint32_t v = sub_map[k][ph];
+                ph = v & 63u;
+                cnt += v >> 6;
+            }
+            maps[f * 64 + lane] = (cnt << 8) | ph;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------- composition
+
+// Group g of stream t covers child maps [64 g, 64 g + 64) of that stream.
+// Map word: (count << 8) | exit phase.
+__global__ __launch_bounds__(64) void k_reduce(const uint64_t *__restrict__ in,
+                                               const uint64_t *__restrict__ in_first,
+                                               const uint64_t *__restrict__ out_first, int T,
+                                               uint64_t ngroups, uint64_t *__restrict__ out) {
+    const int lane = threadIdx.x;
+    for (uint64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        const int t = find_seg(out_first, T, gi);
+        const uint64_t g = gi - out_first[t];
+        const uint64_t nin = in_first[t + 1] - in_first[t];
+        const uint64_t c0 = g * 64;
+        const int nc = (int)min<uint64_t>(64, nin - c0);
+        const uint64_t *base = in + (in_first[t] + c0) * 64;
+        uint64_t m[64];
+#pragma unroll
+        for (int k = 0; k < 64; ++k) m[k] = k < nc ? base[k * 64 + lane] : 0;
+        uint32_t ph = (uint32_t)lane;
+        uint64_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 64; ++k) {
+            if (k < nc) {
+                const uint64_t v = __shfl(m[k], (int)ph);
+                ph = (uint32_t)(v & 255u);
+                cnt += v >> 8;
+            }
+        }
+        out[gi * 64 + lane] = (cnt << 8) | ph;
+    }
+}
+
+// Entries: (first candidate index << 8) | entry phase.
+__global__ __launch_bounds__(64) void k_down(const uint64_t *__restrict__ in,
+                                             const uint64_t *__restrict__ in_first,
+                                             const uint64_t *__restrict__ out_first, int T,
+                                             uint64_t ngroups,
+                                             const uint64_t *__restrict__ parent_entry,
+                                             uint64_t *__restrict__ child_entry) {
+    const int lane = threadIdx.x;
+    for (uint64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        const int t = find_seg(out_first, T, gi);
+        const uint64_t g = gi - out_first[t];
+        const uint64_t nin = in_first[t + 1] - in_first[t];
+        const uint64_t c0 = g * 64;
+        const int nc = (int)min<uint64_t>(64, nin - c0);
+        const uint64_t *base = in + (in_first[t] + c0) * 64;
+        uint64_t m[64];
+#pragma unroll
+        for (int k = 0; k < 64; ++k) m[k] = k < nc ? base[k * 64 + lane] : 0;
+        const uint64_t pe = parent_entry[gi];
+        uint32_t ph = (uint32_t)(pe & 255u);
+        uint64_t idx = pe >> 8;
+        uint64_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < 64; ++k) {
+            if (k < nc) {
+                if (lane == k) mine = (idx << 8) | ph;
+                const uint64_t v = __shfl(m[k], (int)ph);
+                ph = (uint32_t)(v & 255u);
+                idx += v >> 8;
+            }
+        }
+        if (lane < nc) child_entry[in_first[t] + c0 + lane] = mine;
+    }
+}
+
+// -------------------------------------------------------------- k_score
+
+constexpr int QCAP = 128;  // per-wave ring of queued candidate starts
+
+struct ScoreArgs {
+    const StreamInfo *streams;
+    const LaunchSeg *segs;
+    const uint64_t *seg_first;  // T+1 prefix of blocks in this launch
+    int T;
+    uint64_t nflat;
+    const uint64_t *entries;  // per block: (first index << 8) | phase
+    const float *elites;      // [ne][29]
+    uint32_t ne;
+    uint64_t ne_magic;
+    float sigma;
+    uint32_t immT;
+    uint32_t jumpT;
+    const DevModel *model;  // device copy, read with scalar loads
+    Record *rec;
+    uint32_t rec_cap;
+    uint32_t *rec_count;
+    unsigned long long *cand_count;
+};
+
+template <bool IMM, bool LDS_ELITES>
+__device__ __forceinline__ void score_batch(const ScoreArgs &a,
+                                            const float *el, const uint32_t *q, uint32_t head,
+                                            uint32_t count, uint64_t bstart, uint64_t first,
+                                            uint64_t seed, uint32_t stream, float bound,
+                                            int lane) {
+    const bool act = (uint32_t)lane < count;
+    const uint32_t code = act ? q[(head + (uint32_t)lane) & (QCAP - 1)] : 0u;
+    const uint64_t c = bstart + (code & 0xffffu);
+    const uint64_t cidx = first + ((code >> 16) & 0x7ffu);
+    const bool jp = (code >> 28) & 1u;
+    uint64_t s = seed + (c + 1) * GAMMA;
+    const uint32_t ia = fastmod(draw_hi(s), a.ne_magic, a.ne);
+    s += GAMMA;
+    const uint32_t ib = fastmod(draw_hi(s), a.ne_magic, a.ne);
+    s += 2 * GAMMA;  // the immigrant test draw is already known
+    float g[CS_NTOT];
+    if (IMM) {
+#pragma unroll
+        for (int k = 0; k < CS_NTOT; ++k) {
+            g[k] = unit_f(draw_u24(s));
+            s += GAMMA;
+        }
+    } else {
+        const float sigma = a.sigma;
+#pragma unroll
+        for (int k = 0; k < CS_NTOT; ++k) {
+            const uint32_t bit = draw_bit(s);
+            s += GAMMA;
+            const uint32_t mu = draw_u24(s);
+            s += GAMMA;
+            const uint32_t e = bit ? ib : ia;
+            const float par = el[e * CS_NTOT + k];
+            float d = (unit_f(mu) - 0.5f) * 2.0f;
+            d = d * sigma;
+            const float v = par + d;
+            g[k] = fmaxf(0.0f, fminf(1.0f, v));
+        }
+    }
+    s += GAMMA;  // the jump test draw is already known
+    if (jp) {
+        const uint32_t gi = draw_hi(s) % (uint32_t)CS_NTOT;
+        s += GAMMA;
+        const float val = unit_f(draw_u24(s));
+#pragma unroll
+        for (int k = 0; k < CS_NTOT; ++k) g[k] = (k == (int)gi) ? val : g[k];
+    }
+    // Model.score: lanes j = 0..15 of acc + sum, ordered sum, then the
+    // linear tail over genes 16..28 (src/screen.zig:97-112).  The weights are
+    // fetched per batch through an opaque uniform pointer so they stream
+    // through scalar loads instead of being pinned (and spilled) for the
+    // whole kernel.
+    uint64_t mp = (uint64_t)a.model;
+    asm volatile("" : "+s"(mp));
+    ConstModel md = (ConstModel)mp;
+    float S = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const float acc = md->w0 + md->lin[j] * g[j];
+        float sum = 0.0f;
+#pragma unroll
+        for (int i = j; i < 16; ++i) sum = sum + md->tri[i * (i + 1) / 2 + j] * (g[j] * g[i]);
+        const float ln = acc + sum;
+        S = (j == 0) ? ln : S + ln;
+    }
+    float r = 0.0f;
+#pragma unroll
+    for (int i = 16; i < CS_NTOT; ++i) r = r + md->lin[i] * g[i];
+    const float score = S + r;
+
+    const bool pass = act && !(score <= bound);
+    const uint64_t pm = __ballot(pass);
+    if (pm) {
+        uint32_t basei = 0;
+        if (lane == 0) basei = atomicAdd(a.rec_count, (uint32_t)__popcll(pm));
+        basei = __shfl(basei, 0);
+        if (pass) {
+            const uint32_t slot = basei + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull));
+            if (slot < a.rec_cap) {
+                Record *o = a.rec + slot;
+                o->score = score;
+                o->stream = stream;
+                o->index = cidx;
+                o->start = c;
+                o->ia = ia;
+                o->ib = ib;
+#pragma unroll
+                for (int k = 0; k < CS_NTOT; ++k) o->g[k] = g[k];
+                o->_pad = 0;
+            }
+        }
+    }
+}
+
+template <bool LDS_ELITES>
+__global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
+    extern __shared__ uint32_t smem[];
+    uint32_t *qbase = smem;  // 4 waves x 2 rings x QCAP
+    float *el_lds = reinterpret_cast<float *>(smem + 4 * 2 * QCAP);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float *el = a.elites;
+    if (LDS_ELITES) {
+        for (uint32_t i = threadIdx.x; i < a.ne * CS_NTOT; i += blockDim.x) el_lds[i] = a.elites[i];
+        __syncthreads();
+        el = el_lds;
+    }
+    uint32_t *qN = qbase + wave * 2 * QCAP;
+    uint32_t *qI = qN + QCAP;
+    unsigned long long screened = 0;
+    for (uint64_t fb = (uint64_t)blockIdx.x * 4 + wave; fb < a.nflat; fb += (uint64_t)gridDim.x * 4) {
+        const int t = find_seg(a.seg_first, a.T, fb);
+        const LaunchSeg sg = a.segs[t];
+        const StreamInfo st = a.streams[t];
+        const uint64_t b = sg.b0 + (fb - sg.flat_first);
+        const uint64_t ent = a.entries[st.blk_first + b];
+        const uint64_t first = ent >> 8;
+        if (first >= st.my_lambda) continue;
+        const uint64_t bstart = b << BLOCK_LOG, bend = bstart + BLOCK;
+        uint64_t c = bstart + (ent & 255u);
+        uint64_t idx = first;
+        uint32_t headN = 0, tailN = 0, headI = 0, tailI = 0;
+        const uint64_t lt = (1ull << lane) - 1ull;
+        while (c < bend && idx < st.my_lambda) {
+            // speculate: lane j starts at c + 62 j (no immigrant, no jump before it)
+            const uint64_t cj = c + 62ull * (uint64_t)lane;
+            const bool im = draw_u24(st.seed + (cj + 3) * GAMMA) < a.immT;
+            const uint64_t pj = cj + (im ? 33u : 62u);
+            const bool jp = draw_u24(st.seed + pj * GAMMA) < a.jumpT;
+            const uint32_t L = im ? (jp ? 35u : 33u) : (jp ? 64u : 62u);
+            const uint64_t stop = __ballot(L != 62u);
+            uint64_t n = stop ? (uint64_t)__ffsll((unsigned long long)stop) : 64ull;
+            const uint64_t room = (bend - c + 61) / 62;
+            const uint64_t lam = st.my_lambda - idx;
+            n = min(n, min(room, lam));
+            const bool take = (uint64_t)lane < n;
+            const uint32_t code = (uint32_t)(cj - bstart) | ((uint32_t)(idx - first + (uint64_t)lane) << 16) |
+                                  ((uint32_t)im << 27) | ((uint32_t)jp << 28);
+            const uint64_t mN = __ballot(take && !im), mI = __ballot(take && im);
+            if (take && !im) qN[(tailN + (uint32_t)__popcll(mN & lt)) & (QCAP - 1)] = code;
+            if (take && im) qI[(tailI + (uint32_t)__popcll(mI & lt)) & (QCAP - 1)] = code;
+            tailN += (uint32_t)__popcll(mN);
+            tailI += (uint32_t)__popcll(mI);
+            const uint32_t Llast = __shfl(L, (int)(n - 1));
+            c += 62ull * (n - 1) + Llast;
+            idx += n;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            while (tailN - headN >= 64) {
+                score_batch<false, LDS_ELITES>(a, el, qN, headN, 64, bstart, first, st.seed, (uint32_t)t,
+                                               sg.bound, lane);
+                headN += 64;
+                screened += 64;
+            }
+            if (tailI - headI >= 64) {
+                score_batch<true, LDS_ELITES>(a, el, qI, headI, 64, bstart, first, st.seed, (uint32_t)t,
+                                              sg.bound, lane);
+                headI += 64;
+                screened += 64;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (tailN > headN) {
+            score_batch<false, LDS_ELITES>(a, el, qN, headN, tailN - headN, bstart, first, st.seed,
+                                           (uint32_t)t, sg.bound, lane);
+            screened += tailN - headN;
+        }
+        if (tailI > headI) {
+            score_batch<true, LDS_ELITES>(a, el, qI, headI, tailI - headI, bstart, first, st.seed,
+                                          (uint32_t)t, sg.bound, lane);
+            screened += tailI - headI;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0 && screened) atomicAdd(a.cand_count, screened);
+}
+
+// ------------------------------------------------------------------ host
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(CS_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+static int ensure(DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes) return CS_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t want = bytes + bytes / 4 + 4096;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(CS_E_OOM, "hipMalloc(" + std::to_string(want) + "): " + hipGetErrorString(e));
+    }
+    b.bytes = want;
+    return CS_OK;
+}
+
+}  // namespace csf
+
+struct cs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    csf::DevBuf streams, blk_first, maps, entries, upper_maps, upper_entries, level_first, elites,
+        segs, seg_first, rec, counters, model;
+    std::vector<csf::Record> host_rec;
+    cs_screen_stats stats{};
+    uint32_t rec_cap = 1u << 20;
+};
+
+namespace csf {
+
+static uint32_t event_threshold(float p) {
+    // number of 24-bit numerators m with (float)m * 2^-24 < p
+    if (!(p > 0.0f)) return 0;
+    double x = std::ceil((double)p * 16777216.0);
+    if (x > 16777216.0) x = 16777216.0;
+    return (uint32_t)x;
+}
+
+static double ms_between(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+    return ms;
+}
+
+static float nan_f() { return std::nanf(""); }
+
+struct HostState {
+    cs_ctx *ctx;
+    int T;
+    std::vector<StreamInfo> si;
+    std::vector<StreamHeap> heaps;
+    const cs_genome *elites;
+    ScoreArgs sa;
+    bool lds_elites;
+    size_t smem;
+    uint64_t records = 0, cand = 0;
+    uint32_t launches = 0, retries = 0;
+    double ms_score = 0, ms_replay = 0;
+};
+
+// Score the block ranges [lo[t], hi[t]) of every stream in one launch and
+// replay the emitted candidates through the stream heaps.  On record-buffer
+// overflow the ranges are split in two and processed in order.
+static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std::vector<uint64_t> &hi) {
+    cs_ctx *ctx = h.ctx;
+    const int T = h.T;
+    std::vector<LaunchSeg> segs(T);
+    std::vector<uint64_t> first(T + 1, 0);
+    for (int t = 0; t < T; ++t) {
+        uint64_t n = hi[t] > lo[t] ? hi[t] - lo[t] : 0;
+        first[t + 1] = first[t] + n;
+        segs[t].flat_first = first[t];
+        segs[t].b0 = lo[t];
+        StreamHeap &hp = h.heaps[t];
+        segs[t].bound = hp.full() ? hp.root() : nan_f();
+        segs[t]._pad = 0;
+    }
+    const uint64_t nflat = first[T];
+    if (nflat == 0) return CS_OK;
+    int rc;
+    if ((rc = ensure(ctx->segs, sizeof(LaunchSeg) * T))) return rc;
+    if ((rc = ensure(ctx->seg_first, sizeof(uint64_t) * (T + 1)))) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->segs.p, segs.data(), sizeof(LaunchSeg) * T, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->seg_first.p, first.data(), sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->counters.p, 0, 16, ctx->stream));
+    ScoreArgs a = h.sa;
+    a.segs = (const LaunchSeg *)ctx->segs.p;
+    a.seg_first = (const uint64_t *)ctx->seg_first.p;
+    a.nflat = nflat;
+    uint64_t grid = (nflat + 3) / 4;
+    if (grid > (1u << 20)) grid = 1u << 20;
+    HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+    if (h.lds_elites)
+        hipLaunchKernelGGL(k_score<true>, dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
+    else
+        hipLaunchKernelGGL(k_score<false>, dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+    uint32_t cnt = 0;
+    unsigned long long cand = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, a.rec_count, 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&cand, a.cand_count, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    h.ms_score += ms_between(ctx->ev[4], ctx->ev[5]);
+    h.launches++;
+    auto t0 = std::chrono::steady_clock::now();
+    if (cnt > ctx->rec_cap) {
+        // overflow: split and redo in order (streams are independent)
+        h.retries++;
+        std::vector<uint64_t> lo1 = lo, hi1 = hi, lo2 = lo, hi2 = hi;
+        int active = 0;
+        for (int t = 0; t < T; ++t) active += hi[t] > lo[t];
+        if (active > 1) {
+            int seen = 0;
+            for (int t = 0; t < T; ++t) {
+                if (hi[t] <= lo[t]) continue;
+                if (seen < active / 2) lo2[t] = hi2[t] = 0;
+                else lo1[t] = hi1[t] = 0;
+                ++seen;
+            }
+        } else {
+            for (int t = 0; t < T; ++t) {
+                if (hi[t] <= lo[t]) continue;
+                uint64_t mid = lo[t] + (hi[t] - lo[t]) / 2;
+                if (mid == lo[t]) return fail(CS_E_INTERNAL, "record buffer too small for one block");
+                hi1[t] = mid;
+                lo2[t] = mid;
+            }
+        }
+        h.ms_replay += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if ((rc = score_ranges(h, lo1, hi1))) return rc;
+        return score_ranges(h, lo2, hi2);
+    }
+    h.records += cnt;
+    h.cand += cand;
+    if (cnt) {
+        if (ctx->host_rec.size() < cnt) ctx->host_rec.resize(cnt);
+        HIPCHK(hipMemcpy(ctx->host_rec.data(), a.rec, sizeof(Record) * cnt, hipMemcpyDeviceToHost));
+        std::sort(ctx->host_rec.begin(), ctx->host_rec.begin() + cnt, [](const Record &x, const Record &y) {
+            return x.stream != y.stream ? x.stream < y.stream : x.index < y.index;
+        });
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const Record &r = ctx->host_rec[i];
+            cs_survivor s;
+            s.score = r.score;
+            std::memcpy(s.g, r.g, sizeof s.g);
+            s.pa = h.elites[r.ia].id;
+            s.pb = h.elites[r.ib].id;
+            s.tribe = h.elites[r.ia].tribe;
+            s.stream = r.stream;
+            s.index = r.index;
+            s.start = r.start;
+            h.heaps[r.stream].offer(s);
+        }
+    }
+    h.ms_replay += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return CS_OK;
+}
+
+static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne,
+                       const cs_screen_cfg *cfg, cs_survivor *out, uint64_t cap, uint64_t *n_out,
+                       uint64_t *screens) {
+    auto wall0 = std::chrono::steady_clock::now();
+    ctx->stats = cs_screen_stats{};
+    if (ne < 2) return fail(CS_E_NEEDTWO, "NeedTwoElites");
+    if (cfg->keep == 0) return fail(CS_E_KEEP0, "keep must be >= 1 (keep=0 is undefined in the reference)");
+    // finite scores: every term is bounded by |w| since genes lie in [0, 1]
+    {
+        const float *w = &model->w0;
+        for (size_t i = 0; i < sizeof(cs_model) / 4; ++i)
+            if (!(std::fabs(w[i]) <= 1e30f))
+                return fail(CS_E_BADMODEL, "model weights must be finite and |w| <= 1e30 on the device path");
+    }
+    const uint64_t lambda = cfg->lambda;
+    uint64_t T64 = std::min<uint64_t>(cfg->threads, lambda);
+    if (T64 < 1) T64 = 1;
+    if (T64 > (1u << 24)) return fail(CS_E_BADARGS, "threads must be <= 2^24");
+    const int T = (int)T64;
+    HostState h;
+    h.ctx = ctx;
+    h.T = T;
+    h.elites = elites;
+    h.si.resize(T);
+    h.heaps.resize(T);
+    const uint32_t immT = event_threshold(cfg->imm_p), jumpT = event_threshold(cfg->jump_p);
+    // draws per candidate: mean and spread, for the block budget
+    const double pi = immT / 16777216.0, pj = jumpT / 16777216.0;
+    const double meanL = pi * (33 + 2 * pj) + (1 - pi) * (62 + 2 * pj);
+    const double varL = pi * (1 - pi) * 29.0 * 29.0 + 4.0 * pj * (1 - pj) + 1.0;
+    const uint64_t per = lambda / T64;
+    uint64_t issued = 0, nblk = 0;
+    std::vector<uint64_t> blk_first(T + 1, 0);
+    for (int t = 0; t < T; ++t) {
+        const uint64_t my = (t == T - 1) ? lambda - issued : per;
+        issued += per;
+        h.si[t].seed = cfg->seed ^ (GAMMA * (uint64_t)(t + 1));
+        h.si[t].my_lambda = my;
+        h.heaps[t].reset(cfg->keep);
+        const double need = (double)my * meanL + 16.0 * std::sqrt((double)my * varL + 1.0) + 128.0;
+        uint64_t nb = (uint64_t)std::ceil(need / (double)BLOCK) + 1;
+        h.si[t].blk_first = nblk;
+        h.si[t].nblocks = nb;
+        blk_first[t] = nblk;
+        nblk += nb;
+    }
+    blk_first[T] = nblk;
+    int rc;
+    // ---- device buffers
+    if ((rc = ensure(ctx->streams, sizeof(StreamInfo) * T))) return rc;
+    if ((rc = ensure(ctx->blk_first, sizeof(uint64_t) * (T + 1)))) return rc;
+    if ((rc = ensure(ctx->maps, sizeof(uint64_t) * 64 * nblk))) return rc;
+    if ((rc = ensure(ctx->entries, sizeof(uint64_t) * nblk))) return rc;
+    if ((rc = ensure(ctx->elites, sizeof(float) * CS_NTOT * (size_t)ne))) return rc;
+    if ((rc = ensure(ctx->rec, sizeof(Record) * ctx->rec_cap))) return rc;
+    if ((rc = ensure(ctx->counters, 64))) return rc;
+    std::vector<float> eg((size_t)ne * CS_NTOT);
+    for (int e = 0; e < ne; ++e) {
+        for (int k = 0; k < CS_NG; ++k) eg[(size_t)e * CS_NTOT + k] = elites[e].g[k];
+        for (int k = 0; k < CS_NR; ++k) eg[(size_t)e * CS_NTOT + CS_NG + k] = elites[e].reg[k];
+    }
+    HIPCHK(hipMemcpyAsync(ctx->streams.p, h.si.data(), sizeof(StreamInfo) * T, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->blk_first.p, blk_first.data(), sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->elites.p, eg.data(), eg.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    // ---- 1. block maps
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    {
+        uint64_t grid = std::min<uint64_t>(nblk, 1u << 22);
+        hipLaunchKernelGGL(k_map, dim3((unsigned)grid), dim3(256), 0, ctx->stream,
+                           (const StreamInfo *)ctx->streams.p, (const uint64_t *)ctx->blk_first.p, T, nblk, immT,
+                           jumpT, (uint64_t *)ctx->maps.p);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    // ---- 2. compose: levels of maps until one map per stream
+    std::vector<std::vector<uint64_t>> lvl_first;  // per level, T+1 prefix
+    lvl_first.push_back(blk_first);
+    std::vector<uint64_t> lvl_off;  // element offset of each upper level in upper buffers
+    uint64_t upper_total = 0;
+    for (;;) {
+        const auto &cur = lvl_first.back();
+        bool done = true;
+        for (int t = 0; t < T; ++t)
+            if (cur[t + 1] - cur[t] > 1) done = false;
+        if (done) break;
+        std::vector<uint64_t> nx(T + 1, 0);
+        for (int t = 0; t < T; ++t) nx[t + 1] = nx[t] + (cur[t + 1] - cur[t] + 63) / 64;
+        lvl_off.push_back(upper_total);
+        upper_total += nx[T];
+        lvl_first.push_back(nx);
+    }
+    const int L = (int)lvl_first.size() - 1;
+    if ((rc = ensure(ctx->upper_maps, sizeof(uint64_t) * 64 * (upper_total + 1)))) return rc;
+    if ((rc = ensure(ctx->upper_entries, sizeof(uint64_t) * (upper_total + 1)))) return rc;
+    if ((rc = ensure(ctx->level_first, sizeof(uint64_t) * (T + 1) * (L + 1)))) return rc;
+    // host staging vectors stay alive until the stream synchronizes below
+    std::vector<uint64_t> flat_first;
+    for (auto &v : lvl_first) flat_first.insert(flat_first.end(), v.begin(), v.end());
+    HIPCHK(hipMemcpyAsync(ctx->level_first.p, flat_first.data(), flat_first.size() * 8, hipMemcpyHostToDevice,
+                          ctx->stream));
+    auto lvl_maps = [&](int l) -> uint64_t * {
+        return l == 0 ? (uint64_t *)ctx->maps.p : (uint64_t *)ctx->upper_maps.p + lvl_off[l - 1] * 64;
+    };
+    auto lvl_entries = [&](int l) -> uint64_t * {
+        return l == 0 ? (uint64_t *)ctx->entries.p : (uint64_t *)ctx->upper_entries.p + lvl_off[l - 1];
+    };
+    auto lvl_firstp = [&](int l) -> const uint64_t * { return (const uint64_t *)ctx->level_first.p + (size_t)l * (T + 1); };
+    for (int l = 0; l < L; ++l) {
+        const uint64_t ng = lvl_first[l + 1][T];
+        unsigned grid = (unsigned)std::min<uint64_t>(ng, 1u << 20);
+        hipLaunchKernelGGL(k_reduce, dim3(grid), dim3(64), 0, ctx->stream, lvl_maps(l), lvl_firstp(l),
+                           lvl_firstp(l + 1), T, ng, lvl_maps(l + 1));
+        HIPCHK(hipGetLastError());
+    }
+    // coverage check: the top map at phase 0 counts every candidate whose
+    // start lies in the mapped blocks
+    std::vector<uint64_t> top(T), top_entries(T, 0);
+    {
+        uint64_t *tm = lvl_maps(L);
+        for (int t = 0; t < T; ++t)
+            HIPCHK(hipMemcpyAsync(&top[t], tm + lvl_first[L][t] * 64, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(lvl_entries(L), top_entries.data(), 8 * T, hipMemcpyHostToDevice, ctx->stream));
+    }
+    for (int l = L - 1; l >= 0; --l) {
+        const uint64_t ng = lvl_first[l + 1][T];
+        unsigned grid = (unsigned)std::min<uint64_t>(ng, 1u << 20);
+        hipLaunchKernelGGL(k_down, dim3(grid), dim3(64), 0, ctx->stream, lvl_maps(l), lvl_firstp(l),
+                           lvl_firstp(l + 1), T, ng, lvl_entries(l + 1), lvl_entries(l));
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int t = 0; t < T; ++t)
+        if ((top[t] >> 8) < h.si[t].my_lambda)
+            return fail(CS_E_INTERNAL, "block budget did not cover the stream's candidates");
+    // ---- 3. scoring launches over geometrically growing block ranges
+    DevModel md;
+    md.w0 = model->w0;
+    for (int k = 0; k < CS_NTOT; ++k) md.lin[k] = model->lin[k];
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j <= i; ++j) md.tri[i * (i + 1) / 2 + j] = model->quad[i * 16 + j];
+    if ((rc = ensure(ctx->model, sizeof(DevModel)))) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->model.p, &md, sizeof(DevModel), hipMemcpyHostToDevice, ctx->stream));
+    ScoreArgs &a = h.sa;
+    a.streams = (const StreamInfo *)ctx->streams.p;
+    a.T = T;
+    a.entries = (const uint64_t *)ctx->entries.p;
+    a.elites = (const float *)ctx->elites.p;
+    a.ne = (uint32_t)ne;
+    a.ne_magic = UINT64_MAX / (uint64_t)ne + 1;
+    a.sigma = cfg->sigma;
+    a.immT = immT;
+    a.jumpT = jumpT;
+    a.model = (const DevModel *)ctx->model.p;
+    a.rec = (Record *)ctx->rec.p;
+    a.rec_cap = ctx->rec_cap;
+    a.rec_count = (uint32_t *)ctx->counters.p;
+    a.cand_count = (unsigned long long *)((char *)ctx->counters.p + 8);
+    const size_t el_bytes = (size_t)ne * CS_NTOT * 4;
+    h.lds_elites = el_bytes <= 48 * 1024;
+    h.smem = 4 * 2 * QCAP * 4 + (h.lds_elites ? el_bytes : 0);
+    {
+        std::vector<uint64_t> lo(T), hi(T);
+        uint64_t maxnb = 0;
+        for (int t = 0; t < T; ++t) maxnb = std::max(maxnb, h.si[t].nblocks);
+        uint64_t a0 = 0, a1 = 16;
+        while (a0 < maxnb) {
+            for (int t = 0; t < T; ++t) {
+                lo[t] = std::min(a0, h.si[t].nblocks);
+                hi[t] = std::min(a1, h.si[t].nblocks);
+            }
+            if ((rc = score_ranges(h, lo, hi))) return rc;
+            a0 = a1;
+            a1 *= 16;
+        }
+    }
+    // ---- 4. merge heaps t-major, stable sort by score descending
+    auto tm0 = std::chrono::steady_clock::now();
+    std::vector<cs_survivor> all;
+    for (int t = 0; t < T; ++t) all.insert(all.end(), h.heaps[t].slots.begin(), h.heaps[t].slots.end());
+    std::stable_sort(all.begin(), all.end(), [](const cs_survivor &x, const cs_survivor &y) { return x.score > y.score; });
+    uint64_t out_n = std::min<uint64_t>(all.size(), cfg->keep);
+    if (out_n > cap) return fail(CS_E_CAP, "survivor buffer too small");
+    std::memcpy(out, all.data(), out_n * sizeof(cs_survivor));
+    *n_out = out_n;
+    *screens = lambda;
+    h.ms_replay += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tm0).count();
+    cs_screen_stats &S = ctx->stats;
+    S.candidates = h.cand;
+    S.positions = nblk * BLOCK;
+    S.blocks = nblk;
+    S.records = h.records;
+    S.levels = h.launches;
+    S.retries = h.retries;
+    S.ms_map = ms_between(ctx->ev[0], ctx->ev[1]);
+    S.ms_scan = ms_between(ctx->ev[1], ctx->ev[2]);
+    S.ms_score = h.ms_score;
+    S.ms_replay = h.ms_replay;
+    S.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    return CS_OK;
+}
+
+}  // namespace csf
+
+using namespace csf;
+
+extern "C" {
+
+int cs_ctx_create(int device, cs_ctx **out) {
+    if (!out) return fail(CS_E_BADARGS, "null argument");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
+        return fail(CS_E_NODEV, "no HIP device " + std::to_string(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(CS_E_NODEV, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+    HIPCHK(hipSetDevice(device));
+    cs_ctx *c = new cs_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(CS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    for (auto &ev : c->ev) (void)hipEventCreate(&ev);
+    *out = c;
+    return CS_OK;
+}
+
+int cs_ctx_destroy(cs_ctx *c) {
+    if (!c) return CS_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    csf::DevBuf *bufs[] = {&c->streams, &c->blk_first, &c->maps, &c->entries, &c->upper_maps, &c->upper_entries,
+                           &c->level_first, &c->elites, &c->segs, &c->seg_first, &c->rec, &c->counters, &c->model};
+    for (auto *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    for (auto &ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CS_OK;
+}
+
+int cs_screen(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne, const cs_screen_cfg *cfg,
+              cs_survivor *out, uint64_t cap, uint64_t *n_out, uint64_t *screens) {
+    if (!ctx || !model || !cfg || !n_out || !screens || (ne > 0 && !elites) || (cap > 0 && !out))
+        return fail(CS_E_BADARGS, "null argument");
+    clear_error();
+    HIPCHK(hipSetDevice(ctx->device));
+    return screen_impl(ctx, model, elites, ne, cfg, out, cap, n_out, screens);
+}
+
+int cs_get_stats(cs_ctx *ctx, cs_screen_stats *out) {
+    if (!ctx || !out) return fail(CS_E_BADARGS, "null argument");
+    *out = ctx->stats;
+    return CS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+/*
+ * csfunnel.h -- C ABI of the MI355X (gfx950) screening funnel for codec-arena.
+ *
+ * The reference has no FFI on this path: brain/evolve.py:431-442 runs the Zig
+ * CLI `arena screen ...` as a subprocess and reads two files back.  Each entry
+ * point below replaces one piece of that CLI; the python package
+ * (codec-arena_amd/csfunnel) binds them with ctypes and provides the
+ * flag-compatible `arena screen` command.
+ *
+ * Conventions: every function returns an int status (CS_OK = 0, negative =
+ * error; cs_last_error() gives a message, per thread).  All buffers are
+ * caller-owned host memory; device memory is owned by a cs_ctx.  Calls are
+ * synchronous.  A cs_ctx is not re-entrant: use one per host thread.
+ */
+#ifndef CSFUNNEL_H
+#define CSFUNNEL_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CS_NG 23   /* effector genes   (src/genome.zig:14) */
+#define CS_NR 6    /* regulator genes  (src/genome.zig:15) */
+#define CS_NTOT 29 /* flat gene vector (src/genome.zig:16) */
+
+#define CS_OK 0
+#define CS_E_BADMODEL (-1)  /* error.BadModel      src/screen.zig:37,120,123 */
+#define CS_E_NEEDTWO (-2)   /* error.NeedTwoElites src/main.zig:214         */
+#define CS_E_BADARGS (-3)   /* error.BadArgs       src/main.zig:165         */
+#define CS_E_IO (-4)        /* file errors / size limits                     */
+#define CS_E_BADLINE (-5)   /* error.BadLine       src/genome.zig:121       */
+#define CS_E_PARSE (-6)     /* parseInt / parseFloat errors                  */
+#define CS_E_KEEP0 (-7)     /* keep == 0: undefined in the reference (R3)    */
+#define CS_E_CAP (-8)       /* caller buffer too small                       */
+#define CS_E_HIP (-9)       /* HIP runtime error                             */
+#define CS_E_OOM (-10)      /* device allocation failed                      */
+#define CS_E_INTERNAL (-11) /* internal consistency check failed             */
+#define CS_E_NODEV (-12)    /* no usable gfx950 device                       */
+
+/* Genome: src/genome.zig:78-86 (regulators default to 0.5). */
+typedef struct cs_genome {
+    uint64_t id;
+    uint32_t gen;
+    uint32_t tribe;
+    uint64_t pa;
+    uint64_t pb;
+    float g[CS_NG];
+    float reg[CS_NR];
+} cs_genome;
+
+/* Flat model: src/screen.zig:23-26 (w_quad lower triangle, strict upper = 0). */
+typedef struct cs_model {
+    float w0;
+    float lin[CS_NTOT];
+    float quad[256];
+} cs_model;
+
+/* src/screen.zig:129-137.  `threads` is semantic: it fixes the RNG stream
+ * partition (src/screen.zig:267-287), not host or device concurrency. */
+typedef struct cs_screen_cfg {
+    uint64_t lambda;
+    uint64_t keep;
+    uint64_t seed;
+    uint64_t threads;
+    float sigma;
+    float jump_p;
+    float imm_p;
+    float _pad;
+} cs_screen_cfg;
+
+/* One survivor in reference output order: the heap Slot of
+ * src/screen.zig:139-149 plus where its candidate came from. */
+typedef struct cs_survivor {
+    float score;
+    float g[CS_NTOT];
+    uint64_t pa;
+    uint64_t pb;
+    uint32_t tribe;
+    uint32_t stream; /* RNG stream (reference worker thread) */
+    uint64_t index;  /* candidate index within the stream */
+    uint64_t start;  /* stream draw counter before the candidate's first draw */
+} cs_survivor;
+
+/* Per-call device statistics of the last cs_screen (milliseconds are wall
+ * time of the device stages measured with HIP events). */
+typedef struct cs_screen_stats {
+    uint64_t candidates;   /* counted by the scoring kernel */
+    uint64_t positions;    /* RNG draws covered by the offset maps */
+    uint64_t blocks;       /* 65536-draw blocks mapped */
+    uint64_t records;      /* candidates that passed the running bound */
+    uint32_t levels;       /* scoring launches (bound refresh points) */
+    uint32_t retries;      /* overflowed launches that were split */
+    double ms_map;         /* offset-map kernel */
+    double ms_scan;        /* map composition (reduce + down sweeps) */
+    double ms_score;       /* synthesis + score + filter kernels */
+    double ms_replay;      /* host heap replay + copies */
+    double ms_total;
+} cs_screen_stats;
+
+typedef struct cs_ctx cs_ctx;
+
+int cs_version(void);
+const char *cs_last_error(void);
+
+/* Zig std.fmt.parseInt(u64, tok, 10) / parseFloat(f32, tok) as used by the
+ * screen CLI (src/main.zig:178-198) and the parsers below.  0 ok, <0 error. */
+int cs_parse_u64(const char *tok, size_t len, uint64_t *out);
+int cs_parse_f32(const char *tok, size_t len, float *out);
+
+/* src/genome.zig:116-188 (including the v6 parse of tokens 21..26 as 0).
+ * Returns 1 = parsed, 0 = skipped line (wrong token count or '#'), <0 error. */
+int cs_parse_pop_line(const char *line, size_t len, cs_genome *out);
+
+/* src/screen.zig:36-95 (Model.fromCompact), n in {153,159,161,164,165,166}. */
+int cs_model_from_compact(const float *w, int n, cs_model *out);
+
+/* src/screen.zig:115-127 (loadModel, file <= 1 MiB). */
+int cs_load_model(const char *path, cs_model *out);
+
+/* src/main.zig:201-213 (elites file <= 4 MiB, bad lines skipped).
+ * *n_out receives the number of genomes; CS_E_CAP if it exceeds cap. */
+int cs_load_elites(const char *path, cs_genome *out, int cap, int *n_out);
+
+/* Device context on HIP device `device`. */
+int cs_ctx_create(int device, cs_ctx **out);
+int cs_ctx_destroy(cs_ctx *ctx);
+
+/* src/screen.zig:257-320 (run): synthesize cfg->lambda candidates in
+ * cfg->threads splitmix64 streams on the GPU, score them with the quadratic
+ * surface and return the top cfg->keep in the reference's output order.
+ * *screens receives the number of candidates scored (== lambda). */
+int cs_screen(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne,
+              const cs_screen_cfg *cfg, cs_survivor *out, uint64_t cap, uint64_t *n_out,
+              uint64_t *screens);
+
+int cs_get_stats(cs_ctx *ctx, cs_screen_stats *out);
+
+/* Zig `{d:.4}` of an f32 (src/screen.zig:327). Returns the length written. */
+int cs_format_gene(float v, char *buf, size_t cap);
+
+/* src/screen.zig:322-337: "0 0 pa pb tribe" + 24 genes at {d:.4} per line. */
+int cs_write_promoted(const char *path, const cs_survivor *s, uint64_t n);
+
+/* src/screen.zig:338-341: "<total>\n". */
+int cs_write_count(const char *path, uint64_t total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

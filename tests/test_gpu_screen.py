@@ -1,0 +1,180 @@
+"""GPU parity of the quadratic screen: the HIP funnel (through the C ABI) must
+return exactly the oracle's survivors -- scores, genes, lineage and stream
+positions bit for bit, in the same order -- and screens_count == lambda."""
+import os
+
+import numpy as np
+import pytest
+
+from csfunnel import screen as scr
+from csfunnel._abi import CsError
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 0x9E3779B97F4A7C15
+
+
+@pytest.fixture(scope="module")
+def dev():
+    s = scr.Screen(0)
+    yield s
+    s.close()
+
+
+def _c0(golden, name="c0_elites.txt", model="c0_model.txt"):
+    return os.path.join(golden, model), os.path.join(golden, name)
+
+
+def _same(gpu, ref):
+    assert len(gpu) == len(ref)
+    for f in ("score", "g", "pa", "pb", "tribe", "stream", "index", "start"):
+        a, b = gpu[f], ref[f]
+        if a.dtype.kind == "f":
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f
+        else:
+            assert np.array_equal(a, b), f
+
+
+def _run_both(dev, oracle, model_path, elites_path, lam, keep, seed, threads=4, sigma=0.12, jump_p=0.02,
+              imm_p=0.01, model_w=None):
+    if model_w is None:
+        m = scr.load_model(model_path)
+        mo = oracle.load_model(model_path)
+    else:
+        m = scr.Model.from_compact(model_w)
+        mo = oracle.model_from_compact(model_w)
+    el = scr.load_elites(elites_path)
+    elo = oracle.load_elites(elites_path)
+    cfg = scr.Cfg(lam, keep, seed, threads, float(np.float32(sigma)), float(np.float32(jump_p)),
+                  float(np.float32(imm_p)))
+    print(f"case lam={lam} keep={keep} seed={seed} T={threads} sigma={sigma} jump={jump_p} imm={imm_p} "
+          f"ne={len(el)}", flush=True)
+    got, screens = dev.run(m, el, cfg)
+    ref, rscreens = oracle.screen(mo, elo, lam, keep, seed, threads, cfg.sigma, cfg.jump_p, cfg.imm_p)
+    assert screens == rscreens == lam
+    _same(got, ref)
+    return got, dev.stats()
+
+
+@pytest.mark.parametrize("threads", [4, 1])
+def test_config0_parity(dev, oracle, golden, threads):
+    m, e = _c0(golden)
+    got, st = _run_both(dev, oracle, m, e, 1 << 20, 256, 0, threads, 0.12, 0.01, 0.01)
+    assert st["candidates"] == 1 << 20
+    assert np.all(np.diff(got["score"]) <= 0)
+
+
+def test_reference_fixtures_parity(dev, oracle, golden):
+    # Makefile `make screen` inputs (161-term model, 96 v6 elites) at 1e6
+    _run_both(dev, oracle, os.path.join(golden, "surrogate_161.txt"), os.path.join(golden, "seed_pop.txt"),
+              1_000_000, 192, 7, 4)
+
+
+@pytest.mark.parametrize("lam,keep,threads", [(1, 5, 4), (3, 5, 4), (7, 3, 8), (1000, 2000, 4), (5000, 1, 3),
+                                              (65536, 64, 64), (300000, 100, 7)])
+def test_sizes_and_partitions(dev, oracle, golden, lam, keep, threads):
+    m, e = _c0(golden)
+    _run_both(dev, oracle, m, e, lam, keep, 12345, threads)
+
+
+@pytest.mark.parametrize("sigma,jump_p,imm_p", [(0.0, 0.0, 0.0), (0.3, 0.0, 1.0), (0.05, 1.0, 0.0),
+                                                (0.5, 0.5, 0.5), (0.12, 0.02, 0.01)])
+def test_rates(dev, oracle, golden, sigma, jump_p, imm_p):
+    m, e = _c0(golden)
+    _run_both(dev, oracle, m, e, 200000, 128, 99, 4, sigma, jump_p, imm_p)
+
+
+@pytest.mark.parametrize("n", [153, 159, 161, 164, 165, 166])
+def test_model_widths(dev, oracle, golden, n):
+    w = (np.random.default_rng(n).standard_normal(n) / np.sqrt(n)).astype(np.float32)
+    _, e = _c0(golden)
+    _run_both(dev, oracle, None, e, 100000, 64, n, 4, model_w=w)
+
+
+def test_all_ties(dev, oracle, golden):
+    # zero model: every score equal; survivors are the first candidates of
+    # each stream in heap order, t-major (exact heap replay)
+    _, e = _c0(golden)
+    got, _ = _run_both(dev, oracle, None, e, 1 << 20, 300, 5, 4, model_w=np.zeros(153, np.float32))
+    assert list(got["stream"][:256]) == [0] * 256
+    # constant intercept only: same ties, different value
+    w = np.zeros(153, np.float32)
+    w[0] = 0.75
+    _run_both(dev, oracle, None, e, 100000, 300, 6, 4, model_w=w)
+
+
+def test_coarse_scores_many_ties(dev, oracle, golden, tmp_path):
+    # weights on a coarse lattice and elites on a coarse lattice with no
+    # mutation: many exactly equal scores straddle the cut
+    rng = np.random.default_rng(8)
+    w = rng.integers(-2, 3, 153).astype(np.float32)
+    pop = tmp_path / "coarse.txt"
+    with open(pop, "w") as f:
+        for i in range(8):
+            genes = " ".join(str(x) for x in rng.integers(0, 2, 16))
+            f.write(f"{i} 0 0 0 {i} {genes}\n")
+    _run_both(dev, oracle, None, str(pop), 400000, 500, 21, 4, sigma=0.0, jump_p=0.0, imm_p=0.0, model_w=w)
+
+
+def test_many_elites_global_path(dev, oracle, golden, tmp_path):
+    rng = np.random.default_rng(2)
+    pop = tmp_path / "big.txt"
+    with open(pop, "w") as f:
+        for i in range(700):
+            genes = " ".join(f"{x:.4f}" for x in rng.random(28))
+            f.write(f"{i} 1 2 3 {i % 12} {genes}\n")  # 33-token v5 lines
+    m, _ = _c0(golden)
+    _run_both(dev, oracle, m, str(pop), 300000, 150, 4, 4)
+
+
+def test_screen_cmd_files_match_oracle(oracle, golden, tmp_path):
+    m, e = _c0(golden)
+    out, cnt = str(tmp_path / "promoted.txt"), str(tmp_path / "screens.txt")
+    rc = scr.screen_cmd(["--model", m, "--elites", e, "--out", out, "--count-out", cnt, "--lambda", "300000",
+                         "--keep", "192", "--seed", "77", "--threads", "4", "--sigma", "0.1200", "--jump", "0.02",
+                         "--imm", "0.01"])
+    assert rc == 0
+    ref, total = oracle.screen(oracle.load_model(m), oracle.load_elites(e), 300000, 192, 77, 4,
+                               float(np.float32(0.12)), float(np.float32(0.02)), float(np.float32(0.01)))
+    oracle.write_promoted(str(tmp_path / "ref.txt"), ref)
+    assert open(out, "rb").read() == open(tmp_path / "ref.txt", "rb").read()
+    assert open(cnt).read() == f"{total}\n" == "300000\n"
+
+
+def test_keep_zero_and_empty(dev, golden):
+    m, e = _c0(golden)
+    with pytest.raises(CsError):
+        dev.run(scr.load_model(m), scr.load_elites(e), scr.Cfg(1000, 0, 1))
+    got, screens = dev.run(scr.load_model(m), scr.load_elites(e), scr.Cfg(0, 10, 1))
+    assert len(got) == 0 and screens == 0
+
+
+def test_large_lambda_parity(dev, oracle, golden):
+    # 2^26 candidates in 4 streams: every survivor bit-exact against the oracle
+    m, e = _c0(golden, "c1_elites.txt")
+    _run_both(dev, oracle, m, e, 1 << 26, 128, 3, 4, 0.12, 0.01, 0.01)
+
+
+def test_production_generation_properties(dev, oracle, golden):
+    # one full 2.65e9-candidate generation (128 elites, keep 128, T=4):
+    # count, order, and every survivor re-synthesized by the oracle from its
+    # (stream, start) and re-scored bit for bit
+    m, e = _c0(golden, "c1_elites.txt")
+    model, el = scr.load_model(m), scr.load_elites(e)
+    mo, elo = oracle.load_model(m), oracle.load_elites(e)
+    lam = 2_650_000_000
+    cfg = scr.Cfg(lam, 128, 0, 4, float(np.float32(0.12)), float(np.float32(0.01)), float(np.float32(0.01)))
+    print("case production generation lam=2.65e9", flush=True)
+    got, screens = dev.run(model, el, cfg)
+    st = dev.stats()
+    assert screens == lam and st["candidates"] == lam
+    assert len(got) == 128 and np.all(np.diff(got["score"]) <= 0)
+    per = lam // 4
+    for s in got:
+        assert s["index"] < (per if s["stream"] < 3 else lam - 3 * per)
+        g, L, ia, ib = oracle.synth(elo, oracle.stream_seed(0, int(s["stream"])), int(s["start"]), cfg.sigma,
+                                    cfg.jump_p, cfg.imm_p)
+        assert np.array_equal(g.view(np.uint32), s["g"].view(np.uint32))
+        assert np.float32(oracle.score(mo, g)).view(np.uint32) == s["score"].view(np.uint32)
+        assert (elo[ia].id, elo[ib].id, elo[ia].tribe) == (s["pa"], s["pb"], s["tribe"])
+    print(f"stats {st}", flush=True)
