@@ -1,0 +1,162 @@
+"""Headline benchmark: candidates screened per second on one MI355X.
+
+Workload (production-shard quadratic screen): 128 elites, a 153-term
+quadratic surface, 2.65e9 candidates per generation in 4 splitmix64 streams,
+keep 128, jump 0.01, immigrants 0.01, sigma_g = "%.4f" % (0.12 * 0.93^g),
+seed_g = g; the survivors of each generation become the next generation's
+elites (ids 128 g + rank).  One step = one generation.  Inputs are generated
+on the device; only the 128 elites and the model cross PCIe per step.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-sample N]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "codec-arena_amd"))
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+LAMBDA = 2_650_000_000
+KEEP = 128
+THREADS = 4
+JUMP = 0.01
+IMM = 0.01
+FLOP_PER_CANDIDATE = 442  # 136 products g_i g_j + 153 weight products + 153 adds
+PEAK_FP32_TFLOPS = 157.3
+PUBLISHED_CPU = 17.2e6    # reference Zig SIMD screen, 4 threads (docs/SCALE.md:26)
+
+
+def sigma_of(g):
+    return float(np.float32(float("%.4f" % (0.12 * 0.93 ** g))))
+
+
+def next_elites(survivors, gen):
+    from csfunnel import _abi
+    n = len(survivors)
+    arr = (_abi.Genome * n)()
+    for r, s in enumerate(survivors):
+        e = arr[r]
+        e.id = gen * 128 + r
+        e.gen = gen + 1
+        e.pa, e.pb, e.tribe = int(s["pa"]), int(s["pb"]), int(s["tribe"])
+        for k in range(23):
+            e.g[k] = float(s["g"][k])
+        for k in range(6):
+            e.reg[k] = float(s["g"][23 + k])
+    return arr
+
+
+def cpu_baseline(sample, workers):
+    """The oracle (C restatement of src/screen.zig) on the host cores."""
+    from oracle_binding import Oracle
+    o = Oracle()
+    golden = os.path.join(REPO, "tests", "golden")
+    m = o.load_model(os.path.join(golden, "c0_model.txt"))
+    el = o.load_elites(os.path.join(golden, "c1_elites.txt"))
+    t0 = time.perf_counter()
+    o.screen(m, el, sample, KEEP, 0, THREADS, sigma_of(0), float(np.float32(JUMP)), float(np.float32(IMM)),
+             workers=workers)
+    dt = time.perf_counter() - t0
+    return sample / dt, dt
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 26)
+    ap.add_argument("--cpu-workers", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the funnel does not shard across GPUs); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from csfunnel import screen as scr
+    golden = os.path.join(REPO, "tests", "golden")
+    model = scr.load_model(os.path.join(golden, "c0_model.txt"))
+    elites = scr.load_elites(os.path.join(golden, "c1_elites.txt"))
+    dev = scr.Screen(0)
+
+    gen = 0
+    def step():
+        nonlocal elites, gen
+        cfg = scr.Cfg(LAMBDA, KEEP, gen, THREADS, sigma_of(gen), float(np.float32(JUMP)), float(np.float32(IMM)))
+        surv, screens = dev.run(model, elites, cfg)
+        assert screens == LAMBDA and len(surv) == KEEP
+        st = dev.stats()
+        assert st["candidates"] == LAMBDA, st
+        elites = next_elites(surv, gen)
+        gen += 1
+        return st
+
+    for _ in range(args.warmup):
+        st = step()
+        print(f"warmup gen {gen - 1}: {st['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(step())
+        print(f"gen {gen - 1}: {stats[-1]['ms_total']:.1f} ms (map {stats[-1]['ms_map']:.1f}, "
+              f"scan {stats[-1]['ms_scan']:.1f}, score {stats[-1]['ms_score']:.1f}, "
+              f"replay {stats[-1]['ms_replay']:.1f}, records {stats[-1]['records']})", file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    dev.close()
+    value = LAMBDA * args.steps / dt
+    ms_score = sum(s["ms_score"] for s in stats)
+    ms_map = sum(s["ms_map"] for s in stats)
+    launches = sum(s["levels"] for s in stats)
+    achieved = FLOP_PER_CANDIDATE * LAMBDA * args.steps / (ms_score * 1e-3) / 1e12
+    out = {
+        "metric": "candidates screened/s on 1 GPU + top-k recall vs FP64 (quad FP32 vs MLP BF16)",
+        "value": value,
+        "unit": "candidates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "production-shard quadratic screen (2.65e9 candidates/generation, 128 elites, "
+                               "153-term surface, keep 128, 4 streams, chained generations)",
+                   "lambda": LAMBDA, "keep": KEEP, "threads": THREADS, "elites": 128, "jump": JUMP, "imm": IMM,
+                   "first_generation": args.warmup, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "kernel": "k_score", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "flop_per_candidate": FLOP_PER_CANDIDATE, "launches": launches,
+                     "avg_launch_ms": ms_score / max(1, launches), "k_map_ms_per_step": ms_map / args.steps,
+                     "k_score_ms_per_step": ms_score / args.steps},
+        "published_cpu_reference": {"value": PUBLISHED_CPU, "unit": "candidates/s", "cores": 4,
+                                    "source": "docs/SCALE.md:26 (Zig SIMD, 4 threads)"},
+    }
+    if not args.no_cpu:
+        v, secs = cpu_baseline(args.cpu_sample, args.cpu_workers)
+        out["cpu_baseline"] = {"value": v, "unit": "candidates/s", "cores": args.cpu_workers, "kind": "port",
+                               "sample": f"{args.cpu_sample} candidates of generation 0 (same elites, model, "
+                                         f"4 streams) in {secs:.1f} s on {cpu_model()}"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
