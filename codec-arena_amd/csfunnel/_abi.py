@@ -51,7 +51,8 @@ class ScreenStats(C.Structure):
     _fields_ = [("candidates", C.c_uint64), ("positions", C.c_uint64), ("blocks", C.c_uint64),
                 ("records", C.c_uint64), ("levels", C.c_uint32), ("retries", C.c_uint32),
                 ("ms_map", C.c_double), ("ms_scan", C.c_double), ("ms_score", C.c_double),
-                ("ms_replay", C.c_double), ("ms_total", C.c_double)]
+                ("ms_replay", C.c_double), ("ms_total", C.c_double), ("offset_path", C.c_uint32),
+                ("walk_fixes", C.c_uint32)]
 
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcsfunnel.so")
@@ -72,6 +73,7 @@ SIGNATURES = {
                             C.POINTER(ScreenCfg), C.POINTER(Survivor), C.c_uint64,
                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "cs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ScreenStats)]),
+    "cs_set_offset_mode": (C.c_int, [C.c_void_p, C.c_int]),
     "cs_format_gene": (C.c_int, [C.c_float, C.c_char_p, C.c_size_t]),
     "cs_write_promoted": (C.c_int, [C.c_char_p, C.POINTER(Survivor), C.c_uint64]),
     "cs_write_count": (C.c_int, [C.c_char_p, C.c_uint64]),

@@ -152,6 +152,10 @@ class Screen:
                               out.ctypes.data_as(C.POINTER(Survivor)), cap, C.byref(n_out), C.byref(screens)))
         return out[: n_out.value].copy(), screens.value
 
+    def set_offset_mode(self, mode: int) -> None:
+        """0 = automatic (speculative walk, verified), 1 = transfer maps."""
+        check(lib().cs_set_offset_mode(self._h, int(mode)))
+
     def stats(self) -> dict:
         s = _abi.ScreenStats()
         check(lib().cs_get_stats(self._h, C.byref(s)))

@@ -275,6 +275,102 @@ __global__ __launch_bounds__(64) void k_down(const uint64_t *__restrict__ in,
     }
 }
 
+// --------------------------------------------------------------- k_walk
+
+// Segments of 2^25 draws (512 blocks).  The true candidate chain of a stream
+// is found segment by segment in parallel: each wavefront starts a chain at a
+// guessed start BURN draws before its segment and follows it with ballot
+// speculation; two chains that ever share a start coincide from then on, and
+// chains started at different points merge after ~3e5 draws on average at
+// the reference's rates.  The guess is verified on the host: segment s is on
+// the true chain iff its first start >= segment start (x_s) equals the first
+// start >= that point found by segment s-1 (y_{s-1}); segment 0 starts at the
+// true stream start.  Unverified segments are re-walked from y_{s-1}.
+constexpr int SEG_LOG = 25;
+constexpr uint64_t SEG = 1ull << SEG_LOG;
+constexpr int SEG_BLOCKS_LOG = SEG_LOG - BLOCK_LOG;  // 512 blocks per segment
+constexpr uint64_t BURN = 1ull << 23;
+
+struct SegOut {
+    uint64_t x;      // first candidate start >= segment start
+    uint64_t y;      // first candidate start >= segment end
+    uint64_t total;  // candidates starting inside the segment
+};
+
+// Follows one stream's chain from `c` (a candidate start) and records, for
+// every block boundary B in [seg_start, seg_end], the first start >= B and
+// the number of starts in [seg_start, that start).
+__device__ void walk_segment(uint64_t seed, uint64_t c, uint64_t seg_start, uint64_t seg_end, uint32_t immT,
+                             uint32_t jumpT, uint64_t *__restrict__ entries_seg, SegOut *__restrict__ out,
+                             int lane) {
+    uint64_t B = seg_start;  // next block boundary to resolve
+    uint64_t r = 0;          // starts >= seg_start seen so far
+    for (;;) {
+        const uint64_t cj = c + 62ull * (uint64_t)lane;
+        const bool im = draw_u24(seed + (cj + 3) * GAMMA) < immT;
+        const bool jp = draw_u24(seed + (cj + (im ? 33u : 62u)) * GAMMA) < jumpT;
+        const uint32_t L = im ? (jp ? 35u : 33u) : (jp ? 64u : 62u);
+        const uint64_t stop = __ballot(L != 62u);
+        const uint64_t n = stop ? (uint64_t)__ffsll((unsigned long long)stop) : 64ull;
+        // valid starts this round: c + 62 j, j < n (ascending)
+        // lanes before the segment start do not count
+        const uint64_t ks = c >= seg_start ? 0 : min(n, (seg_start - c + 61) / 62);
+        // a round spans < 4032 draws, so at most one boundary falls inside it
+        if (B <= seg_end) {
+            const uint64_t kb = c >= B ? 0 : (B - c + 61) / 62;
+            if (kb < n) {
+                const uint64_t first = c + 62ull * kb;
+                const uint64_t rel = r + (kb - min(kb, ks));
+                if (B == seg_end) {
+                    if (lane == 0) {
+                        out->y = first;
+                        out->total = rel;
+                    }
+                    return;
+                }
+                if (B == seg_start && lane == 0) out->x = first;
+                if (lane == 0) entries_seg[(B - seg_start) >> BLOCK_LOG] = (rel << 8) | (first - B);
+                B += BLOCK;
+            }
+        }
+        r += n - ks;
+        const uint32_t Llast = __shfl(L, (int)(n - 1));
+        c += 62ull * (n - 1) + Llast;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_walk(const StreamInfo *__restrict__ streams,
+                                             const uint64_t *__restrict__ seg_first, int T, uint64_t nseg,
+                                             uint32_t immT, uint32_t jumpT, uint64_t *__restrict__ entries,
+                                             SegOut *__restrict__ segout, int64_t fix_flat, uint64_t fix_start) {
+    const int lane = threadIdx.x;
+    for (uint64_t f = blockIdx.x; f < nseg; f += gridDim.x) {
+        const uint64_t ff = fix_flat >= 0 ? (uint64_t)fix_flat : f;
+        const int t = find_seg(seg_first, T, ff);
+        const uint64_t s = ff - seg_first[t];
+        const StreamInfo st = streams[t];
+        const uint64_t seg_start = s << SEG_LOG;
+        const uint64_t seg_end = min(seg_start + SEG, st.nblocks << BLOCK_LOG);
+        uint64_t c;
+        if (fix_flat >= 0) c = fix_start;
+        else c = s == 0 ? 0 : seg_start - BURN;
+        walk_segment(st.seed, c, seg_start, seg_end, immT, jumpT, entries + st.blk_first + (s << SEG_BLOCKS_LOG),
+                     segout + ff, lane);
+        if (fix_flat >= 0) break;
+    }
+}
+
+// entries[b] += segment base index (per stream prefix of segment totals)
+__global__ void k_rebase(const StreamInfo *__restrict__ streams, const uint64_t *__restrict__ blk_first, int T,
+                         uint64_t nblk, const uint64_t *__restrict__ seg_first,
+                         const uint64_t *__restrict__ seg_base, uint64_t *__restrict__ entries) {
+    for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nblk; f += (uint64_t)gridDim.x * blockDim.x) {
+        const int t = find_seg(blk_first, T, f);
+        const uint64_t b = f - blk_first[t];
+        entries[f] += seg_base[seg_first[t] + (b >> SEG_BLOCKS_LOG)] << 8;
+    }
+}
+
 // -------------------------------------------------------------- k_score
 
 constexpr int QCAP = 128;  // per-wave ring of queued candidate starts
@@ -512,10 +608,11 @@ struct cs_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     csf::DevBuf streams, blk_first, maps, entries, upper_maps, upper_entries, level_first, elites,
-        segs, seg_first, rec, counters, model;
+        segs, seg_first, rec, counters, model, seg_first_buf, segout, seg_base;
     std::vector<csf::Record> host_rec;
     cs_screen_stats stats{};
     uint32_t rec_cap = 1u << 20;
+    bool force_maps = false;  // testing hook: always use the transfer maps
 };
 
 namespace csf {
@@ -546,6 +643,8 @@ struct HostState {
     bool lds_elites;
     size_t smem;
     uint64_t records = 0, cand = 0;
+    uint32_t walk_fixes = 0;
+    bool walked = false;
     uint32_t launches = 0, retries = 0;
     double ms_score = 0, ms_replay = 0;
 };
@@ -650,71 +749,12 @@ static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std
     return CS_OK;
 }
 
-static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne,
-                       const cs_screen_cfg *cfg, cs_survivor *out, uint64_t cap, uint64_t *n_out,
-                       uint64_t *screens) {
-    auto wall0 = std::chrono::steady_clock::now();
-    ctx->stats = cs_screen_stats{};
-    if (ne < 2) return fail(CS_E_NEEDTWO, "NeedTwoElites");
-    if (cfg->keep == 0) return fail(CS_E_KEEP0, "keep must be >= 1 (keep=0 is undefined in the reference)");
-    // finite scores: every term is bounded by |w| since genes lie in [0, 1]
-    {
-        const float *w = &model->w0;
-        for (size_t i = 0; i < sizeof(cs_model) / 4; ++i)
-            if (!(std::fabs(w[i]) <= 1e30f))
-                return fail(CS_E_BADMODEL, "model weights must be finite and |w| <= 1e30 on the device path");
-    }
-    const uint64_t lambda = cfg->lambda;
-    uint64_t T64 = std::min<uint64_t>(cfg->threads, lambda);
-    if (T64 < 1) T64 = 1;
-    if (T64 > (1u << 24)) return fail(CS_E_BADARGS, "threads must be <= 2^24");
-    const int T = (int)T64;
-    HostState h;
-    h.ctx = ctx;
-    h.T = T;
-    h.elites = elites;
-    h.si.resize(T);
-    h.heaps.resize(T);
-    const uint32_t immT = event_threshold(cfg->imm_p), jumpT = event_threshold(cfg->jump_p);
-    // draws per candidate: mean and spread, for the block budget
-    const double pi = immT / 16777216.0, pj = jumpT / 16777216.0;
-    const double meanL = pi * (33 + 2 * pj) + (1 - pi) * (62 + 2 * pj);
-    const double varL = pi * (1 - pi) * 29.0 * 29.0 + 4.0 * pj * (1 - pj) + 1.0;
-    const uint64_t per = lambda / T64;
-    uint64_t issued = 0, nblk = 0;
-    std::vector<uint64_t> blk_first(T + 1, 0);
-    for (int t = 0; t < T; ++t) {
-        const uint64_t my = (t == T - 1) ? lambda - issued : per;
-        issued += per;
-        h.si[t].seed = cfg->seed ^ (GAMMA * (uint64_t)(t + 1));
-        h.si[t].my_lambda = my;
-        h.heaps[t].reset(cfg->keep);
-        const double need = (double)my * meanL + 16.0 * std::sqrt((double)my * varL + 1.0) + 128.0;
-        uint64_t nb = (uint64_t)std::ceil(need / (double)BLOCK) + 1;
-        h.si[t].blk_first = nblk;
-        h.si[t].nblocks = nb;
-        blk_first[t] = nblk;
-        nblk += nb;
-    }
-    blk_first[T] = nblk;
+// Exact offsets from transfer maps of every block (any rates).
+static int offsets_by_maps(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::vector<uint64_t> &blk_first,
+                           uint32_t immT, uint32_t jumpT) {
+    const int T = h.T;
     int rc;
-    // ---- device buffers
-    if ((rc = ensure(ctx->streams, sizeof(StreamInfo) * T))) return rc;
-    if ((rc = ensure(ctx->blk_first, sizeof(uint64_t) * (T + 1)))) return rc;
     if ((rc = ensure(ctx->maps, sizeof(uint64_t) * 64 * nblk))) return rc;
-    if ((rc = ensure(ctx->entries, sizeof(uint64_t) * nblk))) return rc;
-    if ((rc = ensure(ctx->elites, sizeof(float) * CS_NTOT * (size_t)ne))) return rc;
-    if ((rc = ensure(ctx->rec, sizeof(Record) * ctx->rec_cap))) return rc;
-    if ((rc = ensure(ctx->counters, 64))) return rc;
-    std::vector<float> eg((size_t)ne * CS_NTOT);
-    for (int e = 0; e < ne; ++e) {
-        for (int k = 0; k < CS_NG; ++k) eg[(size_t)e * CS_NTOT + k] = elites[e].g[k];
-        for (int k = 0; k < CS_NR; ++k) eg[(size_t)e * CS_NTOT + CS_NG + k] = elites[e].reg[k];
-    }
-    HIPCHK(hipMemcpyAsync(ctx->streams.p, h.si.data(), sizeof(StreamInfo) * T, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->blk_first.p, blk_first.data(), sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice,
-                          ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->elites.p, eg.data(), eg.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     // ---- 1. block maps
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     {
@@ -786,6 +826,154 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     for (int t = 0; t < T; ++t)
         if ((top[t] >> 8) < h.si[t].my_lambda)
             return fail(CS_E_INTERNAL, "block budget did not cover the stream's candidates");
+    return CS_OK;
+}
+
+// Offsets from the speculative segment walk; *ok = false when too many
+// segments failed verification (the caller then uses the maps).
+static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::vector<uint64_t> &blk_first,
+                           uint32_t immT, uint32_t jumpT, bool *ok) {
+    const int T = h.T;
+    int rc;
+    *ok = false;
+    std::vector<uint64_t> seg_first(T + 1, 0);
+    for (int t = 0; t < T; ++t) {
+        const uint64_t nb = h.si[t].nblocks;
+        seg_first[t + 1] = seg_first[t] + ((nb + (1ull << SEG_BLOCKS_LOG) - 1) >> SEG_BLOCKS_LOG);
+    }
+    const uint64_t nseg = seg_first[T];
+    if ((rc = ensure(ctx->seg_first_buf, sizeof(uint64_t) * (T + 1)))) return rc;
+    if ((rc = ensure(ctx->segout, sizeof(SegOut) * nseg))) return rc;
+    if ((rc = ensure(ctx->seg_base, sizeof(uint64_t) * nseg))) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->seg_first_buf.p, seg_first.data(), 8 * (T + 1), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    {
+        unsigned grid = (unsigned)std::min<uint64_t>(nseg, 1u << 20);
+        hipLaunchKernelGGL(k_walk, dim3(grid), dim3(64), 0, ctx->stream, (const StreamInfo *)ctx->streams.p,
+                           (const uint64_t *)ctx->seg_first_buf.p, T, nseg, immT, jumpT, (uint64_t *)ctx->entries.p,
+                           (SegOut *)ctx->segout.p, (int64_t)-1, (uint64_t)0);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+    std::vector<SegOut> so(nseg);
+    HIPCHK(hipMemcpyAsync(so.data(), ctx->segout.p, sizeof(SegOut) * nseg, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // verify the chain segment by segment; re-walk segments that started on
+    // another chain from the verified exit of their predecessor
+    uint32_t fixes = 0;
+    for (int t = 0; t < T; ++t) {
+        for (uint64_t f = seg_first[t] + 1; f < seg_first[t + 1]; ++f) {
+            if (so[f].x == so[f - 1].y) continue;
+            if (++fixes > 32) return CS_OK;  // *ok stays false
+            hipLaunchKernelGGL(k_walk, dim3(1), dim3(64), 0, ctx->stream, (const StreamInfo *)ctx->streams.p,
+                               (const uint64_t *)ctx->seg_first_buf.p, T, (uint64_t)1, immT, jumpT,
+                               (uint64_t *)ctx->entries.p, (SegOut *)ctx->segout.p, (int64_t)f, so[f - 1].y);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(&so[f], (SegOut *)ctx->segout.p + f, sizeof(SegOut), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            if (so[f].x != so[f - 1].y) return fail(CS_E_INTERNAL, "segment fixup did not start on the chain");
+        }
+    }
+    std::vector<uint64_t> base(nseg, 0);
+    for (int t = 0; t < T; ++t) {
+        uint64_t acc = 0;
+        for (uint64_t f = seg_first[t]; f < seg_first[t + 1]; ++f) {
+            base[f] = acc;
+            acc += so[f].total;
+        }
+        if (acc < h.si[t].my_lambda) return fail(CS_E_INTERNAL, "block budget did not cover the stream's candidates");
+    }
+    HIPCHK(hipMemcpyAsync(ctx->seg_base.p, base.data(), 8 * nseg, hipMemcpyHostToDevice, ctx->stream));
+    {
+        unsigned grid = (unsigned)std::min<uint64_t>((nblk + 255) / 256, 1u << 16);
+        hipLaunchKernelGGL(k_rebase, dim3(grid), dim3(256), 0, ctx->stream, (const StreamInfo *)ctx->streams.p,
+                           (const uint64_t *)ctx->blk_first.p, T, nblk, (const uint64_t *)ctx->seg_first_buf.p,
+                           (const uint64_t *)ctx->seg_base.p, (uint64_t *)ctx->entries.p);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    h.walk_fixes = fixes;
+    *ok = true;
+    return CS_OK;
+}
+
+static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne,
+                       const cs_screen_cfg *cfg, cs_survivor *out, uint64_t cap, uint64_t *n_out,
+                       uint64_t *screens) {
+    auto wall0 = std::chrono::steady_clock::now();
+    ctx->stats = cs_screen_stats{};
+    if (ne < 2) return fail(CS_E_NEEDTWO, "NeedTwoElites");
+    if (cfg->keep == 0) return fail(CS_E_KEEP0, "keep must be >= 1 (keep=0 is undefined in the reference)");
+    // finite scores: every term is bounded by |w| since genes lie in [0, 1]
+    {
+        const float *w = &model->w0;
+        for (size_t i = 0; i < sizeof(cs_model) / 4; ++i)
+            if (!(std::fabs(w[i]) <= 1e30f))
+                return fail(CS_E_BADMODEL, "model weights must be finite and |w| <= 1e30 on the device path");
+    }
+    const uint64_t lambda = cfg->lambda;
+    uint64_t T64 = std::min<uint64_t>(cfg->threads, lambda);
+    if (T64 < 1) T64 = 1;
+    if (T64 > (1u << 24)) return fail(CS_E_BADARGS, "threads must be <= 2^24");
+    const int T = (int)T64;
+    HostState h;
+    h.ctx = ctx;
+    h.T = T;
+    h.elites = elites;
+    h.si.resize(T);
+    h.heaps.resize(T);
+    const uint32_t immT = event_threshold(cfg->imm_p), jumpT = event_threshold(cfg->jump_p);
+    // draws per candidate: mean and spread, for the block budget
+    const double pi = immT / 16777216.0, pj = jumpT / 16777216.0;
+    const double meanL = pi * (33 + 2 * pj) + (1 - pi) * (62 + 2 * pj);
+    const double varL = pi * (1 - pi) * 29.0 * 29.0 + 4.0 * pj * (1 - pj) + 1.0;
+    const uint64_t per = lambda / T64;
+    uint64_t issued = 0, nblk = 0;
+    std::vector<uint64_t> blk_first(T + 1, 0);
+    for (int t = 0; t < T; ++t) {
+        const uint64_t my = (t == T - 1) ? lambda - issued : per;
+        issued += per;
+        h.si[t].seed = cfg->seed ^ (GAMMA * (uint64_t)(t + 1));
+        h.si[t].my_lambda = my;
+        h.heaps[t].reset(cfg->keep);
+        const double need = (double)my * meanL + 16.0 * std::sqrt((double)my * varL + 1.0) + 128.0;
+        uint64_t nb = (uint64_t)std::ceil(need / (double)BLOCK) + 1;
+        h.si[t].blk_first = nblk;
+        h.si[t].nblocks = nb;
+        blk_first[t] = nblk;
+        nblk += nb;
+    }
+    blk_first[T] = nblk;
+    int rc;
+    // ---- device buffers
+    if ((rc = ensure(ctx->streams, sizeof(StreamInfo) * T))) return rc;
+    if ((rc = ensure(ctx->blk_first, sizeof(uint64_t) * (T + 1)))) return rc;
+    if ((rc = ensure(ctx->entries, sizeof(uint64_t) * nblk))) return rc;
+    if ((rc = ensure(ctx->elites, sizeof(float) * CS_NTOT * (size_t)ne))) return rc;
+    if ((rc = ensure(ctx->rec, sizeof(Record) * ctx->rec_cap))) return rc;
+    if ((rc = ensure(ctx->counters, 64))) return rc;
+    std::vector<float> eg((size_t)ne * CS_NTOT);
+    for (int e = 0; e < ne; ++e) {
+        for (int k = 0; k < CS_NG; ++k) eg[(size_t)e * CS_NTOT + k] = elites[e].g[k];
+        for (int k = 0; k < CS_NR; ++k) eg[(size_t)e * CS_NTOT + CS_NG + k] = elites[e].reg[k];
+    }
+    HIPCHK(hipMemcpyAsync(ctx->streams.p, h.si.data(), sizeof(StreamInfo) * T, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->blk_first.p, blk_first.data(), sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->elites.p, eg.data(), eg.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    // ---- 1-2. candidate offsets: speculative walk when the event rates make
+    // chains merge quickly, the transfer maps otherwise (or on failure)
+    const uint32_t walk_min = (uint32_t)(0.004 * 16777216.0);
+    bool walked = false;
+    if (immT >= walk_min && jumpT >= walk_min && immT < (1u << 24) && jumpT < (1u << 24) && !ctx->force_maps) {
+        if ((rc = offsets_by_walk(ctx, h, nblk, blk_first, immT, jumpT, &walked))) return rc;
+    }
+    if (!walked) {
+        if ((rc = offsets_by_maps(ctx, h, nblk, blk_first, immT, jumpT))) return rc;
+    }
+    h.walked = walked;
     // ---- 3. scoring launches over geometrically growing block ranges
     DevModel md;
     md.w0 = model->w0;
@@ -850,6 +1038,8 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     S.ms_score = h.ms_score;
     S.ms_replay = h.ms_replay;
     S.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    S.offset_path = h.walked ? 1u : 0u;
+    S.walk_fixes = h.walk_fixes;
     return CS_OK;
 }
 
@@ -886,7 +1076,8 @@ int cs_ctx_destroy(cs_ctx *c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     csf::DevBuf *bufs[] = {&c->streams, &c->blk_first, &c->maps, &c->entries, &c->upper_maps, &c->upper_entries,
-                           &c->level_first, &c->elites, &c->segs, &c->seg_first, &c->rec, &c->counters, &c->model};
+                           &c->level_first, &c->elites, &c->segs, &c->seg_first, &c->rec, &c->counters, &c->model,
+                           &c->seg_first_buf, &c->segout, &c->seg_base};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &ev : c->ev)
@@ -903,6 +1094,12 @@ int cs_screen(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int n
     clear_error();
     HIPCHK(hipSetDevice(ctx->device));
     return screen_impl(ctx, model, elites, ne, cfg, out, cap, n_out, screens);
+}
+
+int cs_set_offset_mode(cs_ctx *ctx, int mode) {
+    if (!ctx || mode < 0 || mode > 1) return fail(CS_E_BADARGS, "offset mode must be 0 (auto) or 1 (maps)");
+    ctx->force_maps = mode == 1;
+    return CS_OK;
 }
 
 int cs_get_stats(cs_ctx *ctx, cs_screen_stats *out) {

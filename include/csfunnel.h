@@ -97,6 +97,8 @@ typedef struct cs_screen_stats {
     double ms_score;       /* synthesis + score + filter kernels */
     double ms_replay;      /* host heap replay + copies */
     double ms_total;
+    uint32_t offset_path;  /* 1 = speculative segment walk, 0 = transfer maps */
+    uint32_t walk_fixes;   /* walk segments re-walked after verification */
 } cs_screen_stats;
 
 typedef struct cs_ctx cs_ctx;
@@ -136,6 +138,12 @@ int cs_screen(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int n
               uint64_t *screens);
 
 int cs_get_stats(cs_ctx *ctx, cs_screen_stats *out);
+
+/* How candidate offsets are recovered: 0 = automatic (speculative walk when
+ * both event rates are >= 0.004, verified, else transfer maps), 1 = always
+ * transfer maps.  Both give identical survivors; exposed for tests and
+ * measurement. */
+int cs_set_offset_mode(cs_ctx *ctx, int mode);
 
 /* Zig `{d:.4}` of an f32 (src/screen.zig:327). Returns the length written. */
 int cs_format_gene(float v, char *buf, size_t cap);

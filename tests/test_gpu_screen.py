@@ -14,9 +14,11 @@ pytestmark = pytest.mark.gpu
 GAMMA = 0x9E3779B97F4A7C15
 
 
-@pytest.fixture(scope="module")
-def dev():
+@pytest.fixture(scope="module", params=["auto", "maps"])
+def dev(request):
     s = scr.Screen(0)
+    s.set_offset_mode(0 if request.param == "auto" else 1)
+    s.mode = request.param
     yield s
     s.close()
 
@@ -47,7 +49,7 @@ def _run_both(dev, oracle, model_path, elites_path, lam, keep, seed, threads=4, 
     elo = oracle.load_elites(elites_path)
     cfg = scr.Cfg(lam, keep, seed, threads, float(np.float32(sigma)), float(np.float32(jump_p)),
                   float(np.float32(imm_p)))
-    print(f"case lam={lam} keep={keep} seed={seed} T={threads} sigma={sigma} jump={jump_p} imm={imm_p} "
+    print(f"case mode={dev.mode} lam={lam} keep={keep} seed={seed} T={threads} sigma={sigma} jump={jump_p} imm={imm_p} "
           f"ne={len(el)}", flush=True)
     got, screens = dev.run(m, el, cfg)
     ref, rscreens = oracle.screen(mo, elo, lam, keep, seed, threads, cfg.sigma, cfg.jump_p, cfg.imm_p)
@@ -61,6 +63,7 @@ def test_config0_parity(dev, oracle, golden, threads):
     m, e = _c0(golden)
     got, st = _run_both(dev, oracle, m, e, 1 << 20, 256, 0, threads, 0.12, 0.01, 0.01)
     assert st["candidates"] == 1 << 20
+    assert st["offset_path"] == (1 if dev.mode == "auto" else 0)
     assert np.all(np.diff(got["score"]) <= 0)
 
 
@@ -168,6 +171,7 @@ def test_production_generation_properties(dev, oracle, golden):
     got, screens = dev.run(model, el, cfg)
     st = dev.stats()
     assert screens == lam and st["candidates"] == lam
+    assert st["offset_path"] == (1 if dev.mode == "auto" else 0)
     assert len(got) == 128 and np.all(np.diff(got["score"]) <= 0)
     per = lam // 4
     for s in got:
@@ -178,3 +182,21 @@ def test_production_generation_properties(dev, oracle, golden):
         assert np.float32(oracle.score(mo, g)).view(np.uint32) == s["score"].view(np.uint32)
         assert (elo[ia].id, elo[ib].id, elo[ia].tribe) == (s["pa"], s["pb"], s["tribe"])
     print(f"stats {st}", flush=True)
+
+
+def test_offset_paths_agree_at_production_size(golden):
+    # the speculative walk (verified) and the transfer maps are two
+    # independent ways to place every candidate; a full generation must give
+    # the same survivors either way
+    m, e = _c0(golden, "c1_elites.txt")
+    model, el = scr.load_model(m), scr.load_elites(e)
+    cfg = scr.Cfg(2_650_000_000, 128, 1, 4, float(np.float32(0.1116)), float(np.float32(0.01)),
+                  float(np.float32(0.01)))
+    out = {}
+    for mode in (0, 1):
+        print(f"case production generation offset mode {mode}", flush=True)
+        with scr.Screen(0) as d:
+            d.set_offset_mode(mode)
+            out[mode], _ = d.run(model, el, cfg)
+            assert d.stats()["offset_path"] == (1 - mode)
+    _same(out[0], out[1])
