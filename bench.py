@@ -30,6 +30,11 @@ IMM = 0.01
 FLOP_PER_CANDIDATE = 442  # 136 products g_i g_j + 153 weight products + 153 adds
 PEAK_FP32_TFLOPS = 157.3
 PUBLISHED_CPU = 17.2e6    # reference Zig SIMD screen, 4 threads (docs/SCALE.md:26)
+# measured once with rocprofv3 (profiles/screen_walk_offsets_2p65e9.md):
+# k_score VALU lane-instructions per candidate and HBM bytes per launch
+VALU_LANE_INSTR = 2222
+VALU_PEAK = 256 * 128 * 2.4e9  # lane-instructions/s: 256 CUs x 128 lanes/cycle x 2.4 GHz
+SCORE_TRAFFIC_PER_LAUNCH = 175.1e6 / 11
 
 
 def sigma_of(g):
@@ -143,7 +148,12 @@ def main():
                    "lambda": LAMBDA, "keep": KEEP, "threads": THREADS, "elites": 128, "jump": JUMP, "imm": IMM,
                    "first_generation": args.warmup, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_score", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": SCORE_TRAFFIC_PER_LAUNCH,
+                     "traffic_source": "profiles/screen_walk_offsets_2p65e9.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+                     "valu_issue": {"lane_instr_per_candidate": VALU_LANE_INSTR,
+                                    "achieved": LAMBDA * args.steps / (ms_score * 1e-3) * VALU_LANE_INSTR,
+                                    "peak": VALU_PEAK, "unit": "lane-instr/s",
+                                    "frac": LAMBDA * args.steps / (ms_score * 1e-3) * VALU_LANE_INSTR / VALU_PEAK},
                      "flop_per_candidate": FLOP_PER_CANDIDATE, "launches": launches,
                      "avg_launch_ms": ms_score / max(1, launches), "k_map_ms_per_step": ms_map / args.steps,
                      "k_score_ms_per_step": ms_score / args.steps},

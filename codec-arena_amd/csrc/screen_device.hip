@@ -386,6 +386,7 @@ struct ScoreArgs {
     uint32_t ne;
     uint64_t ne_magic;
     float sigma;
+    float sig24;  // sigma * 2^-24 (exact when the fast mutation path is used)
     uint32_t immT;
     uint32_t jumpT;
     const DevModel *model;  // device copy, read with scalar loads
@@ -395,7 +396,7 @@ struct ScoreArgs {
     unsigned long long *cand_count;
 };
 
-template <bool IMM, bool LDS_ELITES>
+template <bool IMM, bool LDS_ELITES, bool FAST_MUT>
 __device__ __forceinline__ void score_batch(const ScoreArgs &a,
                                             const float *el, const uint32_t *q, uint32_t head,
                                             uint32_t count, uint64_t bstart, uint64_t first,
@@ -419,7 +420,7 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a,
             s += GAMMA;
         }
     } else {
-        const float sigma = a.sigma;
+        const float sigma = a.sigma, sig24 = a.sig24;
 #pragma unroll
         for (int k = 0; k < CS_NTOT; ++k) {
             const uint32_t bit = draw_bit(s);
@@ -428,10 +429,19 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a,
             s += GAMMA;
             const uint32_t e = bit ? ib : ia;
             const float par = el[e * CS_NTOT + k];
-            float d = (unit_f(mu) - 0.5f) * 2.0f;
-            d = d * sigma;
-            const float v = par + d;
-            g[k] = fmaxf(0.0f, fminf(1.0f, v));
+            if (FAST_MUT) {
+                // ((u - 0.5) * 2) * sigma with u = mu 2^-24 equals
+                // (2 mu - 2^24) * (sigma 2^-24) before the one rounding when
+                // sigma 2^-24 is exact; the clamp of a finite sum to [0, 1]
+                // is one median-of-three.
+                const float d = (float)((int32_t)(mu << 1) - (1 << 24)) * sig24;
+                g[k] = __builtin_amdgcn_fmed3f(par + d, 0.0f, 1.0f);
+            } else {
+                float d = (unit_f(mu) - 0.5f) * 2.0f;
+                d = d * sigma;
+                const float v = par + d;
+                g[k] = fmaxf(0.0f, fminf(1.0f, v));
+            }
         }
     }
     s += GAMMA;  // the jump test draw is already known
@@ -489,7 +499,7 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a,
     }
 }
 
-template <bool LDS_ELITES>
+template <bool LDS_ELITES, bool FAST_MUT>
 __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
     extern __shared__ uint32_t smem[];
     uint32_t *qbase = smem;  // 4 waves x 2 rings x QCAP
@@ -544,13 +554,13 @@ __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             while (tailN - headN >= 64) {
-                score_batch<false, LDS_ELITES>(a, el, qN, headN, 64, bstart, first, st.seed, (uint32_t)t,
+                score_batch<false, LDS_ELITES, FAST_MUT>(a, el, qN, headN, 64, bstart, first, st.seed, (uint32_t)t,
                                                sg.bound, lane);
                 headN += 64;
                 screened += 64;
             }
             if (tailI - headI >= 64) {
-                score_batch<true, LDS_ELITES>(a, el, qI, headI, 64, bstart, first, st.seed, (uint32_t)t,
+                score_batch<true, LDS_ELITES, FAST_MUT>(a, el, qI, headI, 64, bstart, first, st.seed, (uint32_t)t,
                                               sg.bound, lane);
                 headI += 64;
                 screened += 64;
@@ -558,12 +568,12 @@ __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
             __builtin_amdgcn_wave_barrier();
         }
         if (tailN > headN) {
-            score_batch<false, LDS_ELITES>(a, el, qN, headN, tailN - headN, bstart, first, st.seed,
+            score_batch<false, LDS_ELITES, FAST_MUT>(a, el, qN, headN, tailN - headN, bstart, first, st.seed,
                                            (uint32_t)t, sg.bound, lane);
             screened += tailN - headN;
         }
         if (tailI > headI) {
-            score_batch<true, LDS_ELITES>(a, el, qI, headI, tailI - headI, bstart, first, st.seed,
+            score_batch<true, LDS_ELITES, FAST_MUT>(a, el, qI, headI, tailI - headI, bstart, first, st.seed,
                                           (uint32_t)t, sg.bound, lane);
             screened += tailI - headI;
         }
@@ -641,6 +651,7 @@ struct HostState {
     const cs_genome *elites;
     ScoreArgs sa;
     bool lds_elites;
+    bool fast_mut;
     size_t smem;
     uint64_t records = 0, cand = 0;
     uint32_t walk_fixes = 0;
@@ -682,10 +693,14 @@ static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std
     uint64_t grid = (nflat + 3) / 4;
     if (grid > (1u << 20)) grid = 1u << 20;
     HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
-    if (h.lds_elites)
-        hipLaunchKernelGGL(k_score<true>, dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
+    if (h.lds_elites && h.fast_mut)
+        hipLaunchKernelGGL((k_score<true, true>), dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
+    else if (h.lds_elites)
+        hipLaunchKernelGGL((k_score<true, false>), dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
+    else if (h.fast_mut)
+        hipLaunchKernelGGL((k_score<false, true>), dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
     else
-        hipLaunchKernelGGL(k_score<false>, dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
+        hipLaunchKernelGGL((k_score<false, false>), dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     uint32_t cnt = 0;
@@ -990,6 +1005,16 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     a.ne = (uint32_t)ne;
     a.ne_magic = UINT64_MAX / (uint64_t)ne + 1;
     a.sigma = cfg->sigma;
+    a.sig24 = cfg->sigma * 0x1p-24f;
+    {
+        // fast mutation needs sigma * 2^-24 exact and finite parents/sigma
+        bool ok = std::isfinite(cfg->sigma) && (double)a.sig24 == (double)cfg->sigma * 0x1p-24;
+        for (int e = 0; e < ne && ok; ++e) {
+            for (int k = 0; k < CS_NG; ++k) ok = ok && std::isfinite(elites[e].g[k]);
+            for (int k = 0; k < CS_NR; ++k) ok = ok && std::isfinite(elites[e].reg[k]);
+        }
+        h.fast_mut = ok;
+    }
     a.immT = immT;
     a.jumpT = jumpT;
     a.model = (const DevModel *)ctx->model.p;
@@ -1004,7 +1029,7 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         std::vector<uint64_t> lo(T), hi(T);
         uint64_t maxnb = 0;
         for (int t = 0; t < T; ++t) maxnb = std::max(maxnb, h.si[t].nblocks);
-        uint64_t a0 = 0, a1 = 16;
+        uint64_t a0 = 0, a1 = 1;
         while (a0 < maxnb) {
             for (int t = 0; t < T; ++t) {
                 lo[t] = std::min(a0, h.si[t].nblocks);
@@ -1012,7 +1037,7 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
             }
             if ((rc = score_ranges(h, lo, hi))) return rc;
             a0 = a1;
-            a1 *= 16;
+            a1 *= 4;
         }
     }
     // ---- 4. merge heaps t-major, stable sort by score descending

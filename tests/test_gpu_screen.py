@@ -200,3 +200,18 @@ def test_offset_paths_agree_at_production_size(golden):
             out[mode], _ = d.run(model, el, cfg)
             assert d.stats()["offset_path"] == (1 - mode)
     _same(out[0], out[1])
+
+
+def test_slow_mutation_path_edge_inputs(dev, oracle, golden, tmp_path):
+    # a subnormal sigma (sigma * 2^-24 not representable) and a non-finite
+    # parent gene (clamp01(NaN) = 1) take the general mutation path
+    m, e = _c0(golden)
+    _run_both(dev, oracle, m, e, 100000, 64, 31, 4, sigma=1e-40, jump_p=0.02, imm_p=0.01)
+    lines = open(e).read().splitlines()
+    toks = lines[0].split()
+    toks[7] = "nan"
+    toks[9] = "inf"
+    pop = tmp_path / "nan.txt"
+    pop.write_text("\n".join([" ".join(toks)] + lines[1:]) + "\n")
+    got, _ = _run_both(dev, oracle, m, str(pop), 100000, 64, 32, 4)
+    assert np.all((got["g"] >= 0) & (got["g"] <= 1))
