@@ -1,11 +1,15 @@
 """`arena` command replacement for brain/evolve.py's `--zig` binary.
 
-Only the `screen` subcommand is served (src/main.zig:98-101 -> screenCmd); the
-oracle subcommands (eval, psnr, msssimcheck, debug) belong to the codec, which
-this package does not implement, and exit with status 1.
+`screen` runs on the GPU (src/main.zig:98-101 -> screenCmd).  evolve.py sends
+every subcommand to the same binary (brain/evolve.py:113,431), so the oracle
+subcommands (eval, psnr, msssimcheck, debug) are forwarded unchanged to the
+codec binary named by the CSFUNNEL_ORACLE_BIN environment variable; without it
+they exit with status 1.
 """
 from __future__ import annotations
 
+import os
+import subprocess
 import sys
 
 from .screen import screen_cmd
@@ -19,7 +23,10 @@ def main(argv=None) -> int:
         return 1
     if argv[0] == "screen":
         return screen_cmd(argv[1:])
-    sys.stderr.write(f"error: subcommand '{argv[0]}' is not provided by csfunnel (screen only)\n")
+    oracle = os.environ.get("CSFUNNEL_ORACLE_BIN")
+    if oracle:
+        return subprocess.run([oracle] + argv).returncode
+    sys.stderr.write(f"error: subcommand '{argv[0]}' needs the codec binary; set CSFUNNEL_ORACLE_BIN\n")
     return 1
 
 
