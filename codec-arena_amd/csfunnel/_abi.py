@@ -74,6 +74,12 @@ SIGNATURES = {
                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "cs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ScreenStats)]),
     "cs_set_offset_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "cs_ridge_gram_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]),
+    "cs_ridge_solve_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p,
+                                     C.c_void_p]),
+    "cs_ridge_loss_grad_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p,
+                                         C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
     "cs_format_gene": (C.c_int, [C.c_float, C.c_char_p, C.c_size_t]),
     "cs_write_promoted": (C.c_int, [C.c_char_p, C.POINTER(Survivor), C.c_uint64]),
     "cs_write_count": (C.c_int, [C.c_char_p, C.c_uint64]),

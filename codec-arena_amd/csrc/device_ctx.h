@@ -1,0 +1,54 @@
+// device_ctx.h -- the cs_ctx device context shared by the screen and the
+// ridge/MLP halves of libcsfunnel (HIP runtime types; device code only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "funnel_internal.h"
+
+namespace csf {
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(CS_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+inline int ensure(DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes) return CS_OK;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    size_t want = bytes + bytes / 4 + 4096;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(CS_E_OOM, "hipMalloc(" + std::to_string(want) + "): " + hipGetErrorString(e));
+    }
+    b.bytes = want;
+    return CS_OK;
+}
+
+}  // namespace csf
+
+struct cs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    csf::DevBuf streams, blk_first, maps, entries, upper_maps, upper_entries, level_first, elites,
+        segs, seg_first, rec, counters, model, seg_first_buf, segout, seg_base;
+    csf::DevBuf gram_part, ridge_part, ridge_scalar;
+    std::vector<csf::Record> host_rec;
+    cs_screen_stats stats{};
+    uint32_t rec_cap = 1u << 20;
+    bool force_maps = false;  // testing hook: always use the transfer maps
+};
+

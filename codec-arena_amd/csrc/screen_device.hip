@@ -33,7 +33,7 @@
 #include <string>
 #include <vector>
 
-#include "funnel_internal.h"
+#include "device_ctx.h"
 
 namespace csf {
 
@@ -584,49 +584,6 @@ __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
 
 // ------------------------------------------------------------------ host
 
-#define HIPCHK(x)                                                                              \
-    do {                                                                                       \
-        hipError_t e_ = (x);                                                                   \
-        if (e_ != hipSuccess)                                                                  \
-            return fail(CS_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));             \
-    } while (0)
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
-
-static int ensure(DevBuf &b, size_t bytes) {
-    if (b.bytes >= bytes) return CS_OK;
-    if (b.p) (void)hipFree(b.p);
-    b.p = nullptr;
-    b.bytes = 0;
-    size_t want = bytes + bytes / 4 + 4096;
-    hipError_t e = hipMalloc(&b.p, want);
-    if (e != hipSuccess) {
-        b.p = nullptr;
-        return fail(CS_E_OOM, "hipMalloc(" + std::to_string(want) + "): " + hipGetErrorString(e));
-    }
-    b.bytes = want;
-    return CS_OK;
-}
-
-}  // namespace csf
-
-struct cs_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[8] = {};
-    csf::DevBuf streams, blk_first, maps, entries, upper_maps, upper_entries, level_first, elites,
-        segs, seg_first, rec, counters, model, seg_first_buf, segout, seg_base;
-    std::vector<csf::Record> host_rec;
-    cs_screen_stats stats{};
-    uint32_t rec_cap = 1u << 20;
-    bool force_maps = false;  // testing hook: always use the transfer maps
-};
-
-namespace csf {
-
 static uint32_t event_threshold(float p) {
     // number of 24-bit numerators m with (float)m * 2^-24 < p
     if (!(p > 0.0f)) return 0;
@@ -1102,7 +1059,8 @@ int cs_ctx_destroy(cs_ctx *c) {
     (void)hipStreamSynchronize(c->stream);
     csf::DevBuf *bufs[] = {&c->streams, &c->blk_first, &c->maps, &c->entries, &c->upper_maps, &c->upper_entries,
                            &c->level_first, &c->elites, &c->segs, &c->seg_first, &c->rec, &c->counters, &c->model,
-                           &c->seg_first_buf, &c->segout, &c->seg_base};
+                           &c->seg_first_buf, &c->segout, &c->seg_base, &c->gram_part, &c->ridge_part,
+                           &c->ridge_scalar};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &ev : c->ev)

@@ -145,6 +145,29 @@ int cs_get_stats(cs_ctx *ctx, cs_screen_stats *out);
  * measurement. */
 int cs_set_offset_mode(cs_ctx *ctx, int mode);
 
+/* ---- FP64 ridge refit (brain/evolve.py:211-268) on the device ----------
+ * Pointers are DEVICE memory (e.g. torch tensors); ng = 16 selects the
+ * 153-feature layout [1, g0..g15, g_i g_j (j<=i)], ng = 29 the 166-feature
+ * layout of brain/evolve.py:211-220.  Calls synchronize the context stream
+ * before returning; the caller must have finished writing the inputs. */
+
+/* X'X (p x p, row-major), X'y (p) and y'y of the feature matrix X of
+ * genes [n][ng] (the products of brain/evolve.py:264-265). */
+int cs_ridge_gram_f64(cs_ctx *ctx, const double *genes, int ng, const double *y, uint64_t n, double *gram,
+                      double *xty, double *yy);
+
+/* w = solve(gram + alpha I, xty + alpha prior) by Cholesky
+ * (brain/evolve.py:266); prior may be NULL (zero). */
+int cs_ridge_solve_f64(cs_ctx *ctx, const double *gram, const double *xty, int p, double alpha,
+                       const double *prior, double *w);
+
+/* No counterpart in the reference: the mean ridge loss
+ *   L(w) = (|X w - y|^2 + alpha |w - prior|^2) / n
+ * whose minimizer is refit_surrogate's solution, and dL/dw, in one pass over
+ * the data.  loss: device double[1]; grad: device double[p]. */
+int cs_ridge_loss_grad_f64(cs_ctx *ctx, const double *genes, int ng, const double *y, uint64_t n,
+                           const double *w, const double *prior, double alpha, double *loss, double *grad);
+
 /* Zig `{d:.4}` of an f32 (src/screen.zig:327). Returns the length written. */
 int cs_format_gene(float v, char *buf, size_t cap);
 
