@@ -88,12 +88,28 @@ SIGNATURES = {
 _lib = None
 
 
+def _bind_one_hip_runtime() -> None:
+    """Make every user of the HIP runtime in this process share one copy.
+
+    torch's ROCm wheel links `libamdhip64.so` / `libhsa-runtime64.so` by their
+    unversioned names from its own lib directory, this library links
+    `libamdhip64.so.7`.  Loading the runtime under the unversioned names first
+    lets both bind to the same copy whichever is imported first (if torch is
+    already loaded these calls return its copy)."""
+    for name in ("libhsa-runtime64.so", "libamdhip64.so"):
+        try:
+            C.CDLL(name, mode=C.RTLD_GLOBAL)
+        except OSError:
+            pass
+
+
 def lib() -> C.CDLL:
     """Load libcsfunnel.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run csfunnel/build.py")
+        _bind_one_hip_runtime()
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -146,7 +146,9 @@ __global__ __launch_bounds__(256) void k_gram(const double *__restrict__ genes, 
 }
 
 // one thread per tile element; sums the workgroup partials in order.
-// Element (lane, r) of tile (ti, tj) is entry (16 ti + 4 (lane / 16) + r, 16 tj + lane % 16).
+// v_mfma_f64_16x16x4f64 leaves D[i][j] in accumulator register r of lane l
+// with i = 4 r + l / 16 and j = l % 16, so element (lane, r) of tile (ti, tj)
+// is entry (16 ti + 4 r + lane / 16, 16 tj + lane % 16).
 __global__ void k_gram_reduce(const double *__restrict__ part, int nwg, int p, double *__restrict__ gram,
                               double *__restrict__ xty, double *__restrict__ yy) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -154,7 +156,7 @@ __global__ void k_gram_reduce(const double *__restrict__ part, int nwg, int p, d
     const int T = idx / 256, el = idx % 256, lane = el / 4, r = el % 4;
     double s = 0.0;
     for (int w = 0; w < nwg; ++w) s += part[(size_t)w * NTILE * 256 + idx];
-    const int i = tile_i(T) * 16 + 4 * (lane >> 4) + r;
+    const int i = tile_i(T) * 16 + 4 * r + (lane >> 4);
     const int j = tile_j(T) * 16 + (lane & 15);
     // diagonal tiles hold both (i, j) and (j, i): take the upper one only, so
     // every output has exactly one writer
