@@ -74,6 +74,19 @@ SIGNATURES = {
                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "cs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ScreenStats)]),
     "cs_set_offset_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "cs_mlp_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "cs_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "cs_mlp_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "cs_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "cs_mlp_set_fp8_scales": (C.c_int, [C.c_void_p, C.c_float, C.c_float]),
+    "cs_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_void_p, C.POINTER(C.c_float)]),
+    "cs_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_float),
+                              C.POINTER(C.c_float)]),
+    "cs_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_float, C.c_float,
+                                    C.POINTER(C.c_float)]),
+    "cs_screen_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(Genome), C.c_int, C.POINTER(ScreenCfg),
+                               C.c_void_p, C.POINTER(Survivor), C.c_uint64, C.POINTER(C.c_uint64),
+                               C.POINTER(C.c_uint64)]),
     "cs_ridge_gram_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
     "cs_ridge_solve_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p,

@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 REPO = os.path.dirname(PKG_ROOT)
 LIB = os.path.join(HERE, "libcsfunnel.so")
 
-SOURCES = ["screen_device.hip", "ridge_device.hip", "funnel_host.cpp"]
+SOURCES = ["screen_device.hip", "ridge_device.hip", "mlp_device.hip", "funnel_host.cpp"]
 ARCH = "gfx950"
 
 

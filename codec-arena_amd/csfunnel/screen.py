@@ -12,6 +12,7 @@ module only marshals arguments.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import sys
 from dataclasses import dataclass
 from typing import List, Optional, Sequence, Tuple
@@ -118,10 +119,13 @@ class Screen:
         h = C.c_void_p()
         check(lib().cs_ctx_create(device, C.byref(h)))
         self._h = h
+        self._children = weakref.WeakSet()  # MLP handles bound to this context
 
     def close(self) -> None:
         if self._h:
-            lib().cs_ctx_destroy(self._h)
+            for ch in list(self._children):
+                ch.close()
+            check(lib().cs_ctx_destroy(self._h))
             self._h = None
 
     def __enter__(self) -> "Screen":

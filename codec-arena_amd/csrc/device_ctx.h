@@ -50,5 +50,18 @@ struct cs_ctx {
     cs_screen_stats stats{};
     uint32_t rec_cap = 1u << 20;
     bool force_maps = false;  // testing hook: always use the transfer maps
+    int n_cu = 256;
+    int n_mlp = 0;  // live cs_mlp handles bound to this context
 };
 
+// Learned surrogate state (mlp_device.hip): FP32 master parameters, AdamW
+// moments, BF16 copies, the fragment layouts the MLP screen reads, and the
+// activations of the last training batch.
+struct cs_mlp {
+    cs_ctx *ctx = nullptr;
+    csf::DevBuf params, grads, m, v, w1b, w2b, w1f, w2f, w2f8;
+    csf::DevBuf xb, z1, h1b, z2, h2b, out, dout, dz2f, dz2b, dh1, dz1f, dz1b, loss_part;
+    int cap = 0;
+    long step = 0;
+    float h_scale = 1.0f, w_scale = 1.0f;
+};

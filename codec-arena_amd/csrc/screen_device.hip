@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -34,6 +35,7 @@
 #include <vector>
 
 #include "device_ctx.h"
+#include "mlp_common.h"
 
 namespace csf {
 
@@ -52,6 +54,7 @@ struct StreamInfo {
     uint64_t my_lambda;  // candidates of this stream
     uint64_t blk_first;  // index of the stream's first block in block arrays
     uint64_t nblocks;    // blocks mapped for this stream
+    uint64_t cand_base;  // candidates of the streams before this one
 };
 
 struct LaunchSeg {  // one stream's block range in a scoring launch
@@ -394,10 +397,105 @@ struct ScoreArgs {
     uint32_t rec_cap;
     uint32_t *rec_count;
     unsigned long long *cand_count;
+    MlpScreenW mlp;       // learned-surrogate weights (MLP scorers)
+    float *dump_genes;    // optional [lambda][29]: every candidate, stream order
+    float *dump_scores;   // optional [lambda]
 };
 
-template <bool IMM, bool LDS_ELITES, bool FAST_MUT>
-__device__ __forceinline__ void score_batch(const ScoreArgs &a,
+// LDS of the MLP scorers (per workgroup): W2 fragments, b1 | b2 | w3, and per
+// wave the candidates' genes (bf16) and their scores.
+constexpr size_t MLP_W2_BF16_BYTES = 8 * 16 * 64 * 16;
+constexpr size_t MLP_W2_FP8_BYTES = 8 * 16 * 64 * 8;
+constexpr size_t MLP_VEC_BYTES = 3 * MLP_H * 4;
+constexpr size_t MLP_WAVE_BYTES = 64 * MLP_IN * 2 + 64 * 4;
+
+enum { SC_QUAD = 0, SC_MLP_BF16 = 1, SC_MLP_FP8 = 2 };
+
+// One wavefront scores its 64 candidates (genes already in xs) with the MLP.
+// Layer 1 computes H1^T = W1^T X^T on v_mfma_f32_16x16x16_bf16, so each
+// lane's accumulators are, for candidate l%16 of a group of 16, the hidden
+// units 16 mt + 4 (l/16) + r; packed two tiles at a time they are exactly the
+// A fragments of layer 2 in the permuted K order of w2_k_of().  Layer 2 runs
+// on v_mfma_f32_16x16x32_{bf16,fp8}; layer 3 is folded in as sum gelu(z2) w3.
+template <int SC>
+__device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const __bf16 *xs, float *sc, const void *w2lds,
+                                          const float *b1s, const float *b2s, const float *w3s, int lane) {
+    const int q = lane >> 4, j = lane & 15;
+    typedef typename std::conditional<SC == SC_MLP_FP8, uint64_t, bf16x8>::type Frag;
+    Frag af[4][8];
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) {
+        const s16x4 bx = *reinterpret_cast<const s16x4 *>(xs + (c4 * 16 + j) * MLP_IN + 4 * q);
+#pragma unroll
+        for (int mt = 0; mt < 16; ++mt) {
+            f32x4 acc = *reinterpret_cast<const f32x4 *>(b1s + 16 * mt + 4 * q);
+            const bf16x4 wa = w.w1f[mt * 64 + lane];
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const s16x4 *>(&wa), bx, acc, 0, 0, 0);
+            float h[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) h[r] = gelu_f(acc[r]);
+            if constexpr (SC == SC_MLP_FP8) {
+                uint32_t pk = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(h[0] * w.h_scale, h[1] * w.h_scale, 0, false);
+                pk = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(h[2] * w.h_scale, h[3] * w.h_scale, (int)pk, true);
+                if (mt & 1) af[c4][mt >> 1] |= (uint64_t)pk << 32;
+                else af[c4][mt >> 1] = (uint64_t)pk;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) af[c4][mt >> 1][(mt & 1) * 4 + r] = (__bf16)h[r];
+            }
+        }
+    }
+    f32x4 part[4];
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) part[c4] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int nt = 0; nt < 16; ++nt) {
+        Frag wb[8];
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) wb[kb] = reinterpret_cast<const Frag *>(w2lds)[(kb * 16 + nt) * 64 + lane];
+        const float b2n = b2s[16 * nt + j], w3n = w3s[16 * nt + j];
+#pragma unroll
+        for (int c4 = 0; c4 < 4; ++c4) {
+            f32x4 acc = SC == SC_MLP_FP8 ? f32x4{0.f, 0.f, 0.f, 0.f} : f32x4{b2n, b2n, b2n, b2n};
+#pragma unroll
+            for (int kb = 0; kb < 8; ++kb) {
+                if constexpr (SC == SC_MLP_FP8)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)af[c4][kb], (long)wb[kb], acc, 0, 0, 0);
+                else
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c4][kb], wb[kb], acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float z = SC == SC_MLP_FP8 ? acc[r] * w.descale + b2n : acc[r];
+                part[c4][r] += gelu_f(z) * w3n;
+            }
+        }
+    }
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v = part[c4][r];
+            v += __shfl_xor(v, 1);
+            v += __shfl_xor(v, 2);
+            v += __shfl_xor(v, 4);
+            v += __shfl_xor(v, 8);
+            if (j == 0) sc[c4 * 16 + 4 * q + r] = v + w.b3;
+        }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return sc[lane];
+}
+
+struct MlpLds {
+    const void *w2;
+    const float *b1, *b2, *w3;
+    __bf16 *xs;
+    float *sc;
+};
+
+template <bool IMM, bool LDS_ELITES, bool FAST_MUT, int SC, bool DUMP>
+__device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml,
                                             const float *el, const uint32_t *q, uint32_t head,
                                             uint32_t count, uint64_t bstart, uint64_t first,
                                             uint64_t seed, uint32_t stream, float bound,
@@ -452,6 +550,21 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a,
 #pragma unroll
         for (int k = 0; k < CS_NTOT; ++k) g[k] = (k == (int)gi) ? val : g[k];
     }
+    float score;
+    if constexpr (SC != SC_QUAD) {
+        bf16x8 x0, x1;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            x0[k] = (__bf16)g[k];
+            x1[k] = (__bf16)g[8 + k];
+        }
+        *reinterpret_cast<bf16x8 *>(ml.xs + lane * MLP_IN) = x0;
+        *reinterpret_cast<bf16x8 *>(ml.xs + lane * MLP_IN + 8) = x1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        score = mlp_wave<SC>(a.mlp, ml.xs, ml.sc, ml.w2, ml.b1, ml.b2, ml.w3, lane);
+    } else {
     // Model.score: lanes j = 0..15 of acc + sum, ordered sum, then the
     // linear tail over genes 16..28 (src/screen.zig:97-112).  The weights are
     // fetched per batch through an opaque uniform pointer so they stream
@@ -473,7 +586,14 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a,
     float r = 0.0f;
 #pragma unroll
     for (int i = 16; i < CS_NTOT; ++i) r = r + md->lin[i] * g[i];
-    const float score = S + r;
+    score = S + r;
+    }
+    if (DUMP && act) {
+        const uint64_t o = a.streams[stream].cand_base + cidx;
+#pragma unroll
+        for (int k = 0; k < CS_NTOT; ++k) a.dump_genes[o * CS_NTOT + k] = g[k];
+        a.dump_scores[o] = score;
+    }
 
     const bool pass = act && !(score <= bound);
     const uint64_t pm = __ballot(pass);
@@ -499,9 +619,9 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a,
     }
 }
 
-template <bool LDS_ELITES, bool FAST_MUT>
+template <bool LDS_ELITES, bool FAST_MUT, int SC, bool DUMP = false>
 __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
-    extern __shared__ uint32_t smem[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
     uint32_t *qbase = smem;  // 4 waves x 2 rings x QCAP
     float *el_lds = reinterpret_cast<float *>(smem + 4 * 2 * QCAP);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -510,6 +630,30 @@ __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
         for (uint32_t i = threadIdx.x; i < a.ne * CS_NTOT; i += blockDim.x) el_lds[i] = a.elites[i];
         __syncthreads();
         el = el_lds;
+    }
+    MlpLds ml{};
+    if constexpr (SC != SC_QUAD) {
+        // after the rings: W2 fragments, then b1 | b2 | w3, then per-wave xs and scores
+        char *base = reinterpret_cast<char *>(smem + 4 * 2 * QCAP);
+        const size_t w2b = SC == SC_MLP_FP8 ? MLP_W2_FP8_BYTES : MLP_W2_BF16_BYTES;
+        const uint4 *src = SC == SC_MLP_FP8 ? reinterpret_cast<const uint4 *>(a.mlp.w2f8)
+                                            : reinterpret_cast<const uint4 *>(a.mlp.w2f);
+        uint4 *dst = reinterpret_cast<uint4 *>(base);
+        for (uint32_t i = threadIdx.x; i < w2b / 16; i += blockDim.x) dst[i] = src[i];
+        float *vec = reinterpret_cast<float *>(base + w2b);
+        for (uint32_t i = threadIdx.x; i < MLP_H; i += blockDim.x) {
+            vec[i] = a.mlp.b1[i];
+            vec[MLP_H + i] = a.mlp.b2[i];
+            vec[2 * MLP_H + i] = a.mlp.w3[i];
+        }
+        char *wv = base + w2b + MLP_VEC_BYTES + (size_t)wave * MLP_WAVE_BYTES;
+        ml.w2 = base;
+        ml.b1 = vec;
+        ml.b2 = vec + MLP_H;
+        ml.w3 = vec + 2 * MLP_H;
+        ml.xs = reinterpret_cast<__bf16 *>(wv);
+        ml.sc = reinterpret_cast<float *>(wv + 64 * MLP_IN * 2);
+        __syncthreads();
     }
     uint32_t *qN = qbase + wave * 2 * QCAP;
     uint32_t *qI = qN + QCAP;
@@ -554,13 +698,13 @@ __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             while (tailN - headN >= 64) {
-                score_batch<false, LDS_ELITES, FAST_MUT>(a, el, qN, headN, 64, bstart, first, st.seed, (uint32_t)t,
+                score_batch<false, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qN, headN, 64, bstart, first, st.seed, (uint32_t)t,
                                                sg.bound, lane);
                 headN += 64;
                 screened += 64;
             }
             if (tailI - headI >= 64) {
-                score_batch<true, LDS_ELITES, FAST_MUT>(a, el, qI, headI, 64, bstart, first, st.seed, (uint32_t)t,
+                score_batch<true, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qI, headI, 64, bstart, first, st.seed, (uint32_t)t,
                                               sg.bound, lane);
                 headI += 64;
                 screened += 64;
@@ -568,12 +712,12 @@ __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
             __builtin_amdgcn_wave_barrier();
         }
         if (tailN > headN) {
-            score_batch<false, LDS_ELITES, FAST_MUT>(a, el, qN, headN, tailN - headN, bstart, first, st.seed,
+            score_batch<false, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qN, headN, tailN - headN, bstart, first, st.seed,
                                            (uint32_t)t, sg.bound, lane);
             screened += tailN - headN;
         }
         if (tailI > headI) {
-            score_batch<true, LDS_ELITES, FAST_MUT>(a, el, qI, headI, tailI - headI, bstart, first, st.seed,
+            score_batch<true, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qI, headI, tailI - headI, bstart, first, st.seed,
                                           (uint32_t)t, sg.bound, lane);
             screened += tailI - headI;
         }
@@ -609,6 +753,7 @@ struct HostState {
     ScoreArgs sa;
     bool lds_elites;
     bool fast_mut;
+    int sc = SC_QUAD;
     size_t smem;
     uint64_t records = 0, cand = 0;
     uint32_t walk_fixes = 0;
@@ -650,14 +795,36 @@ static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std
     uint64_t grid = (nflat + 3) / 4;
     if (grid > (1u << 20)) grid = 1u << 20;
     HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
-    if (h.lds_elites && h.fast_mut)
-        hipLaunchKernelGGL((k_score<true, true>), dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
-    else if (h.lds_elites)
-        hipLaunchKernelGGL((k_score<true, false>), dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
-    else if (h.fast_mut)
-        hipLaunchKernelGGL((k_score<false, true>), dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
-    else
-        hipLaunchKernelGGL((k_score<false, false>), dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
+    if (h.sc != SC_QUAD) {
+        // persistent: the workgroup loads the MLP weights into LDS once
+        const uint64_t per_cu = std::max<uint64_t>(1, (160u * 1024u) / h.smem);
+        grid = std::min<uint64_t>(grid, (uint64_t)ctx->n_cu * per_cu);
+    }
+    auto go = [&](auto kern) {
+        if (h.smem > 64 * 1024)
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)h.smem);
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
+    };
+    if (a.dump_genes) {  // evaluation runs: every candidate's genes and score
+        if (h.sc == SC_MLP_BF16) go(k_score<false, false, SC_MLP_BF16, true>);
+        else if (h.sc == SC_MLP_FP8) go(k_score<false, false, SC_MLP_FP8, true>);
+        else go(k_score<false, false, SC_QUAD, true>);
+    } else if (h.sc == SC_MLP_BF16) {
+        if (h.fast_mut) go(k_score<false, true, SC_MLP_BF16>);
+        else go(k_score<false, false, SC_MLP_BF16>);
+    } else if (h.sc == SC_MLP_FP8) {
+        if (h.fast_mut) go(k_score<false, true, SC_MLP_FP8>);
+        else go(k_score<false, false, SC_MLP_FP8>);
+    } else if (h.lds_elites && h.fast_mut) {
+        go(k_score<true, true, SC_QUAD>);
+    } else if (h.lds_elites) {
+        go(k_score<true, false, SC_QUAD>);
+    } else if (h.fast_mut) {
+        go(k_score<false, true, SC_QUAD>);
+    } else {
+        go(k_score<false, false, SC_QUAD>);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
     uint32_t cnt = 0;
@@ -872,14 +1039,19 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
 }
 
 static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne,
-                       const cs_screen_cfg *cfg, cs_survivor *out, uint64_t cap, uint64_t *n_out,
-                       uint64_t *screens) {
+                       const cs_screen_cfg *cfg, const cs_screen_opts *opts, cs_survivor *out, uint64_t cap,
+                       uint64_t *n_out, uint64_t *screens) {
     auto wall0 = std::chrono::steady_clock::now();
     ctx->stats = cs_screen_stats{};
     if (ne < 2) return fail(CS_E_NEEDTWO, "NeedTwoElites");
     if (cfg->keep == 0) return fail(CS_E_KEEP0, "keep must be >= 1 (keep=0 is undefined in the reference)");
+    const int sc = opts ? opts->scorer : SC_QUAD;
+    if (sc < SC_QUAD || sc > SC_MLP_FP8) return fail(CS_E_BADARGS, "unknown scorer");
+    if (sc != SC_QUAD && (!opts->mlp || opts->mlp->ctx != ctx))
+        return fail(CS_E_BADARGS, "the MLP scorers need an MLP created on this context");
+    if (sc == SC_QUAD && !model) return fail(CS_E_BADARGS, "the quadratic scorer needs a model");
     // finite scores: every term is bounded by |w| since genes lie in [0, 1]
-    {
+    if (sc == SC_QUAD) {
         const float *w = &model->w0;
         for (size_t i = 0; i < sizeof(cs_model) / 4; ++i)
             if (!(std::fabs(w[i]) <= 1e30f))
@@ -909,6 +1081,7 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         issued += per;
         h.si[t].seed = cfg->seed ^ (GAMMA * (uint64_t)(t + 1));
         h.si[t].my_lambda = my;
+        h.si[t].cand_base = per * (uint64_t)t;
         h.heaps[t].reset(cfg->keep);
         const double need = (double)my * meanL + 16.0 * std::sqrt((double)my * varL + 1.0) + 128.0;
         uint64_t nb = (uint64_t)std::ceil(need / (double)BLOCK) + 1;
@@ -947,13 +1120,39 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     }
     h.walked = walked;
     // ---- 3. scoring launches over geometrically growing block ranges
-    DevModel md;
-    md.w0 = model->w0;
-    for (int k = 0; k < CS_NTOT; ++k) md.lin[k] = model->lin[k];
-    for (int i = 0; i < 16; ++i)
-        for (int j = 0; j <= i; ++j) md.tri[i * (i + 1) / 2 + j] = model->quad[i * 16 + j];
+    DevModel md{};
+    if (sc == SC_QUAD) {
+        md.w0 = model->w0;
+        for (int k = 0; k < CS_NTOT; ++k) md.lin[k] = model->lin[k];
+        for (int i = 0; i < 16; ++i)
+            for (int j = 0; j <= i; ++j) md.tri[i * (i + 1) / 2 + j] = model->quad[i * 16 + j];
+    }
     if ((rc = ensure(ctx->model, sizeof(DevModel)))) return rc;
     HIPCHK(hipMemcpyAsync(ctx->model.p, &md, sizeof(DevModel), hipMemcpyHostToDevice, ctx->stream));
+    h.sc = sc;
+    h.sa.mlp = MlpScreenW{};
+    h.sa.dump_genes = opts ? opts->dump_genes : nullptr;
+    h.sa.dump_scores = opts ? opts->dump_scores : nullptr;
+    if ((h.sa.dump_genes == nullptr) != (h.sa.dump_scores == nullptr))
+        return fail(CS_E_BADARGS, "dump_genes and dump_scores go together");
+    if (sc != SC_QUAD) {
+        cs_mlp *ml = opts->mlp;
+        const float *pp = (const float *)ml->params.p;
+        float b3 = 0.0f;
+        HIPCHK(hipMemcpyAsync(&b3, pp + P_B3, 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        MlpScreenW &mw = h.sa.mlp;
+        mw.w1f = (const bf16x4 *)ml->w1f.p;
+        mw.w2f = (const bf16x8 *)ml->w2f.p;
+        mw.w2f8 = (const uint64_t *)ml->w2f8.p;
+        mw.b1 = pp + P_B1;
+        mw.b2 = pp + P_B2;
+        mw.w3 = pp + P_W3;
+        mw.b3 = b3;
+        mw.h_scale = ml->h_scale;
+        mw.w_scale = ml->w_scale;
+        mw.descale = 1.0f / (ml->h_scale * ml->w_scale);
+    }
     ScoreArgs &a = h.sa;
     a.streams = (const StreamInfo *)ctx->streams.p;
     a.T = T;
@@ -980,8 +1179,10 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     a.rec_count = (uint32_t *)ctx->counters.p;
     a.cand_count = (unsigned long long *)((char *)ctx->counters.p + 8);
     const size_t el_bytes = (size_t)ne * CS_NTOT * 4;
-    h.lds_elites = el_bytes <= 48 * 1024;
+    h.lds_elites = sc == SC_QUAD && el_bytes <= 48 * 1024 && h.sa.dump_genes == nullptr;
     h.smem = 4 * 2 * QCAP * 4 + (h.lds_elites ? el_bytes : 0);
+    if (sc != SC_QUAD)
+        h.smem += (sc == SC_MLP_FP8 ? MLP_W2_FP8_BYTES : MLP_W2_BF16_BYTES) + MLP_VEC_BYTES + 4 * MLP_WAVE_BYTES;
     {
         std::vector<uint64_t> lo(T), hi(T);
         uint64_t maxnb = 0;
@@ -1043,6 +1244,7 @@ int cs_ctx_create(int device, cs_ctx **out) {
     HIPCHK(hipSetDevice(device));
     cs_ctx *c = new cs_ctx();
     c->device = device;
+    c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -1055,6 +1257,7 @@ int cs_ctx_create(int device, cs_ctx **out) {
 
 int cs_ctx_destroy(cs_ctx *c) {
     if (!c) return CS_OK;
+    if (c->n_mlp > 0) return fail(CS_E_BADARGS, "context still has live MLP handles; destroy them first");
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     csf::DevBuf *bufs[] = {&c->streams, &c->blk_first, &c->maps, &c->entries, &c->upper_maps, &c->upper_entries,
@@ -1076,7 +1279,16 @@ int cs_screen(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int n
         return fail(CS_E_BADARGS, "null argument");
     clear_error();
     HIPCHK(hipSetDevice(ctx->device));
-    return screen_impl(ctx, model, elites, ne, cfg, out, cap, n_out, screens);
+    return screen_impl(ctx, model, elites, ne, cfg, nullptr, out, cap, n_out, screens);
+}
+
+int cs_screen_ex(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne, const cs_screen_cfg *cfg,
+                 const cs_screen_opts *opts, cs_survivor *out, uint64_t cap, uint64_t *n_out, uint64_t *screens) {
+    if (!ctx || !cfg || !opts || !n_out || !screens || (ne > 0 && !elites) || (cap > 0 && !out))
+        return fail(CS_E_BADARGS, "null argument");
+    clear_error();
+    HIPCHK(hipSetDevice(ctx->device));
+    return screen_impl(ctx, model, elites, ne, cfg, opts, out, cap, n_out, screens);
 }
 
 int cs_set_offset_mode(cs_ctx *ctx, int mode) {

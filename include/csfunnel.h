@@ -127,6 +127,7 @@ int cs_load_elites(const char *path, cs_genome *out, int cap, int *n_out);
 
 /* Device context on HIP device `device`. */
 int cs_ctx_create(int device, cs_ctx **out);
+/* Fails with CS_E_BADARGS while MLP handles created on ctx are alive. */
 int cs_ctx_destroy(cs_ctx *ctx);
 
 /* src/screen.zig:257-320 (run): synthesize cfg->lambda candidates in
@@ -144,6 +145,48 @@ int cs_get_stats(cs_ctx *ctx, cs_screen_stats *out);
  * transfer maps.  Both give identical survivors; exposed for tests and
  * measurement. */
 int cs_set_offset_mode(cs_ctx *ctx, int mode);
+
+/* ---- Learned surrogate (no counterpart in the reference) --------------
+ * A 16 -> 256 -> 256 -> 1 GELU MLP over the 16 legacy effector genes, trained
+ * on the device with BF16 MFMA forward/backward kernels (FP32 accumulation,
+ * FP32 master weights, AdamW as torch.optim.AdamW) and used as an alternative
+ * screen scorer in BF16 or FP8 (E4M3, per-tensor scales). Gene and target
+ * pointers are DEVICE memory: genes [n][ldg] f32 (the first 16 columns are
+ * the inputs), targets [n] f32. */
+typedef struct cs_mlp cs_mlp;
+#define CS_MLP_PARAMS 70401 /* W1[16][256], b1[256], W2[256][256], b2, w3, b3 */
+
+int cs_mlp_create(cs_ctx *ctx, uint64_t seed, cs_mlp **out);
+int cs_mlp_destroy(cs_mlp *mlp);
+int cs_mlp_get_params(cs_mlp *mlp, float *host_params);
+int cs_mlp_set_params(cs_mlp *mlp, const float *host_params);
+/* FP8 screen scales: H1 and W2 are multiplied by these before E4M3 rounding */
+int cs_mlp_set_fp8_scales(cs_mlp *mlp, float h_scale, float w_scale);
+/* out[n] (device) = MLP(genes); *h1max (optional) = max |H1| for FP8 calibration */
+int cs_mlp_forward(cs_mlp *mlp, const float *genes, int ldg, uint64_t n, float *out, float *h1max);
+/* loss = mean (MLP(genes) - y)^2 and its gradient (host_grads, optional) */
+int cs_mlp_grad(cs_mlp *mlp, const float *genes, int ldg, const float *y, int n, float *loss, float *host_grads);
+/* one AdamW step (beta 0.9 / 0.999, eps 1e-8) on a batch; *loss = batch MSE before the step */
+int cs_mlp_train_step(cs_mlp *mlp, const float *genes, int ldg, const float *y, int n, float lr, float wd,
+                      float *loss);
+
+#define CS_SCORER_QUAD 0     /* src/screen.zig:97-112, exact */
+#define CS_SCORER_MLP_BF16 1 /* learned surrogate, BF16 MFMA */
+#define CS_SCORER_MLP_FP8 2  /* learned surrogate, FP8 MFMA for layer 2 */
+
+typedef struct cs_screen_opts {
+    int scorer;        /* CS_SCORER_* */
+    int _pad;
+    cs_mlp *mlp;       /* required by the MLP scorers */
+    float *dump_genes;  /* optional DEVICE [lambda][29]: every candidate's genes, stream order */
+    float *dump_scores; /* DEVICE [lambda], required with dump_genes: every candidate's score */
+} cs_screen_opts;
+
+/* cs_screen with a choice of scorer and optional dumps; the synthesis,
+ * stream partition, heaps and output order are those of cs_screen.  model may
+ * be NULL for the MLP scorers. */
+int cs_screen_ex(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne, const cs_screen_cfg *cfg,
+                 const cs_screen_opts *opts, cs_survivor *out, uint64_t cap, uint64_t *n_out, uint64_t *screens);
 
 /* ---- FP64 ridge refit (brain/evolve.py:211-268) on the device ----------
  * Pointers are DEVICE memory (e.g. torch tensors); ng = 16 selects the
