@@ -109,12 +109,14 @@ class MLP:
         return loss.value
 
     def calibrate_fp8(self, genes, headroom: float = 224.0) -> Tuple[float, float]:
-        """Per-tensor E4M3 scales: max |H1| and max |W2| map to `headroom`
-        (half of E4M3's 448, leaving room for candidates beyond the sample)."""
+        """Per-tensor E4M3 scales: the largest powers of two that keep max |H1|
+        and max |W2| within `headroom` (half of E4M3's 448, leaving room for
+        candidates beyond the sample)."""
         hmax = max(self.h1_max(genes), 1e-12)
         W2 = unpack(self.params)[2]
         wmax = max(float(np.abs(W2).max()), 1e-12)
-        hs, ws = headroom / hmax, headroom / wmax
+        p2 = lambda x: float(2.0 ** min(15, max(-15, int(np.floor(np.log2(x))))))
+        hs, ws = p2(headroom / hmax), p2(headroom / wmax)
         check(lib().cs_mlp_set_fp8_scales(self._h, C.c_float(hs), C.c_float(ws)))
         return hs, ws
 

@@ -45,6 +45,34 @@ __device__ __forceinline__ float gelu_grad_f(float x) {
     return s + x * s * (1.0f - s) * du2;
 }
 
+// Packed-f16 GELU of the MLP screen (two activations per lane and
+// instruction, no transcendentals).  gelu(x) = x S(x) with
+//   x16 = f16(x) (toward zero), v = x16 q, w = min(v v - 1, 1),
+//   S = clamp(1/2 + v R(w), 0, 1), R of degree 6 in Horner form;
+// q = f16(sqrt 2) / 4 maps |x| <= 4.0004 onto w in [-1, 1], beyond which the
+// FMA's clamp bit saturates S.  A power-of-two scale carried by x (FP8 screen)
+// is absorbed into q.  Coefficients and error bounds: gelu_poly_fit.py
+// (max |S - S_exact| 6.8e-4 with every step in f16).
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+constexpr float GELU_Q = 0.353515625f;  // f16(sqrt 2) / 4, exact in f16
+__device__ __forceinline__ h2 h2s(_Float16 c) { return h2{c, c}; }
+__device__ __forceinline__ h2 pk_rtz(float a, float b) { return __builtin_bit_cast(h2, __builtin_amdgcn_cvt_pkrtz(a, b)); }
+__device__ __forceinline__ h2 gelu_sig_h2(h2 x, h2 q) {
+    const h2 v = x * q;
+    const h2 w = __builtin_elementwise_min(__builtin_elementwise_fma(v, v, h2s(-1.0f16)), h2s(1.0f16));
+    h2 p = h2s(0.022354126f16);
+    p = __builtin_elementwise_fma(p, w, h2s(-0.0514831543f16));
+    p = __builtin_elementwise_fma(p, w, h2s(0.0627441406f16));
+    p = __builtin_elementwise_fma(p, w, h2s(-0.0956420898f16));
+    p = __builtin_elementwise_fma(p, w, h2s(0.157470703f16));
+    p = __builtin_elementwise_fma(p, w, h2s(-0.239624023f16));
+    p = __builtin_elementwise_fma(p, w, h2s(0.497802734f16));
+    const h2 sg = __builtin_elementwise_fma(v, p, h2s(0.5f16));
+    return __builtin_elementwise_min(__builtin_elementwise_max(sg, h2s(0.0f16)), h2s(1.0f16));
+}
+
 // Layouts prepared for the screen (one wavefront scores 64 candidates):
 //   w1f  [16 m-tiles][64 lanes] x 4 bf16: A fragments of W1^T for
 //        v_mfma_f32_16x16x16_bf16 (lane l: hidden 16 m + l%16, genes 4(l/16)..+3)
@@ -58,13 +86,15 @@ __host__ __device__ constexpr int w2_k_of(int kb, int q, int e) {
 
 struct MlpScreenW {
     const bf16x4 *w1f;     // 16 * 64
+    const bf16x4 *w1fh;    // 16 * 64, times h_scale (fp8 screen)
     const bf16x8 *w2f;     // 8 * 16 * 64 (bf16)
     const uint64_t *w2f8;  // 8 * 16 * 64 (fp8 x 8)
     const float *b1, *b2, *w3;
     float b3;
-    float h_scale;   // fp8: H1 is multiplied by this before conversion
-    float w_scale;   // fp8: W2 was multiplied by this before conversion
+    float h_scale;   // fp8: power of two; W1, b1 (so H1) are multiplied by it
+    float w_scale;   // fp8: power of two; W2 was multiplied by it before conversion
     float descale;   // 1 / (h_scale * w_scale)
+    float q1;        // v = x q1 in the layer-1 GELU: GELU_Q / h_scale (fp8) or GELU_Q
 };
 
 }  // namespace csf

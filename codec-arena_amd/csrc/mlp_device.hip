@@ -180,16 +180,20 @@ __global__ void k_adamw(float *__restrict__ p, const float *__restrict__ g, floa
 
 // BF16 copies for training and the fragment layouts for the screen
 __global__ void k_refresh(const float *__restrict__ p, __bf16 *__restrict__ w1b, __bf16 *__restrict__ w2b,
-                          bf16x4 *__restrict__ w1f, bf16x8 *__restrict__ w2f, uint64_t *__restrict__ w2f8,
-                          float w_scale) {
+                          bf16x4 *__restrict__ w1f, bf16x4 *__restrict__ w1fh, bf16x8 *__restrict__ w2f,
+                          uint64_t *__restrict__ w2f8, float h_scale, float w_scale) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < MLP_IN * MLP_H) w1b[i] = (__bf16)p[P_W1 + i];
     if (i < MLP_H * MLP_H) w2b[i] = (__bf16)p[P_W2 + i];
     if (i < 16 * 64) {  // w1f[mt][lane]: hidden 16 mt + lane%16, genes 4 (lane/16) + e
         const int mt = i / 64, lane = i % 64, h = 16 * mt + (lane & 15), k0 = 4 * (lane >> 4);
-        bf16x4 v;
-        for (int e = 0; e < 4; ++e) v[e] = (__bf16)p[P_W1 + (k0 + e) * MLP_H + h];
+        bf16x4 v, vh;
+        for (int e = 0; e < 4; ++e) {
+            v[e] = (__bf16)p[P_W1 + (k0 + e) * MLP_H + h];
+            vh[e] = (__bf16)((float)v[e] * h_scale);  // exact: h_scale is a power of two
+        }
         w1f[i] = v;
+        w1fh[i] = vh;
     }
     if (i < 8 * 16 * 64) {  // w2f[kb][nt][lane]: out unit 16 nt + lane%16, permuted k slots
         const int kb = i / 1024, nt = (i / 64) % 16, lane = i % 64, q = lane >> 4;
@@ -235,8 +239,8 @@ static void launch_gemm(hipStream_t s, int M, int N, int K, const __bf16 *A, lon
 static int refresh(cs_mlp *m) {
     cs_ctx *ctx = m->ctx;
     hipLaunchKernelGGL(k_refresh, dim3(MLP_H * MLP_H / 256), dim3(256), 0, ctx->stream, (const float *)m->params.p,
-                       (__bf16 *)m->w1b.p, (__bf16 *)m->w2b.p, (bf16x4 *)m->w1f.p, (bf16x8 *)m->w2f.p,
-                       (uint64_t *)m->w2f8.p, m->w_scale);
+                       (__bf16 *)m->w1b.p, (__bf16 *)m->w2b.p, (bf16x4 *)m->w1f.p, (bf16x4 *)m->w1fh.p,
+                       (bf16x8 *)m->w2f.p, (uint64_t *)m->w2f8.p, m->h_scale, m->w_scale);
     HIPCHK(hipGetLastError());
     return CS_OK;
 }
@@ -334,7 +338,8 @@ int mlp_setup(cs_mlp *m, uint64_t seed) {
     if ((rc = ensure(m->params, 4 * P_TOTAL)) || (rc = ensure(m->grads, 4 * P_TOTAL)) ||
         (rc = ensure(m->m, 4 * P_TOTAL)) || (rc = ensure(m->v, 4 * P_TOTAL)) ||
         (rc = ensure(m->w1b, 2 * MLP_IN * MLP_H)) || (rc = ensure(m->w2b, 2 * MLP_H * MLP_H)) ||
-        (rc = ensure(m->w1f, 8 * 16 * 64)) || (rc = ensure(m->w2f, 16 * 8 * 16 * 64)) ||
+        (rc = ensure(m->w1f, 8 * 16 * 64)) || (rc = ensure(m->w1fh, 8 * 16 * 64)) ||
+        (rc = ensure(m->w2f, 16 * 8 * 16 * 64)) ||
         (rc = ensure(m->w2f8, 8 * 8 * 16 * 64)))
         return rc;
     std::vector<float> p;
@@ -366,7 +371,7 @@ int cs_mlp_create(cs_ctx *ctx, uint64_t seed, cs_mlp **out) {
 int cs_mlp_destroy(cs_mlp *m) {
     if (!m) return CS_OK;
     (void)hipStreamSynchronize(m->ctx->stream);
-    csf::DevBuf *bufs[] = {&m->params, &m->grads, &m->m, &m->v, &m->w1b, &m->w2b, &m->w1f, &m->w2f, &m->w2f8,
+    csf::DevBuf *bufs[] = {&m->params, &m->grads, &m->m, &m->v, &m->w1b, &m->w2b, &m->w1f, &m->w1fh, &m->w2f, &m->w2f8,
                            &m->xb, &m->z1, &m->h1b, &m->z2, &m->h2b, &m->out, &m->dout, &m->dz2f, &m->dz2b,
                            &m->dh1, &m->dz1f, &m->dz1b, &m->loss_part};
     for (auto *b : bufs)
@@ -393,7 +398,10 @@ int cs_mlp_set_params(cs_mlp *m, const float *host) {
 }
 
 int cs_mlp_set_fp8_scales(cs_mlp *m, float h_scale, float w_scale) {
-    if (!m || !(h_scale > 0) || !(w_scale > 0)) return fail(CS_E_BADARGS, "scales must be positive");
+    int eh = 0, ew = 0;
+    if (!m || !(h_scale > 0) || !(w_scale > 0) || std::frexp(h_scale, &eh) != 0.5f ||
+        std::frexp(w_scale, &ew) != 0.5f || eh < -14 || eh > 16 || ew < -14 || ew > 16)
+        return fail(CS_E_BADARGS, "fp8 scales must be powers of two in [2^-15, 2^15]");
     m->h_scale = h_scale;
     m->w_scale = w_scale;
     int rc;

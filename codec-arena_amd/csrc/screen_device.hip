@@ -406,69 +406,103 @@ struct ScoreArgs {
 // wave the candidates' genes (bf16) and their scores.
 constexpr size_t MLP_W2_BF16_BYTES = 8 * 16 * 64 * 16;
 constexpr size_t MLP_W2_FP8_BYTES = 8 * 16 * 64 * 8;
-constexpr size_t MLP_VEC_BYTES = 3 * MLP_H * 4;
-constexpr size_t MLP_WAVE_BYTES = 64 * MLP_IN * 2 + 64 * 4;
+constexpr size_t MLP_VEC_BYTES = 2 * MLP_H * 4 + (MLP_H / 2) * 4;  // b1 | b2 | w3 as f16 pairs
+constexpr size_t MLP_WAVE_BYTES = 64 * MLP_IN * 2;  // genes (bf16), then the scores over them
+// waves per workgroup: the MLP variants run 8 (two per SIMD) around one LDS
+// copy of W2; the quadratic scorer runs 4
+template <int SC> constexpr int score_waves() { return SC == 0 ? 4 : 8; }
 
 enum { SC_QUAD = 0, SC_MLP_BF16 = 1, SC_MLP_FP8 = 2 };
+
+struct MlpLds {
+    const void *w2;        // W2 fragments (bf16 or fp8)
+    const bf16x4 *w1f;     // W1^T fragments, global (fp8 screen: times h_scale)
+    const float *b1, *b2;  // biases (fp8 screen: b1 h_scale, b2 h_scale w_scale)
+    const uint32_t *w3h;   // w3 as f16 pairs {w3[32 p + j], w3[32 p + 16 + j]} at 16 p + j
+    __bf16 *xs;
+    float *sc;
+};
 
 // One wavefront scores its 64 candidates (genes already in xs) with the MLP.
 // Layer 1 computes H1^T = W1^T X^T on v_mfma_f32_16x16x16_bf16, so each
 // lane's accumulators are, for candidate l%16 of a group of 16, the hidden
 // units 16 mt + 4 (l/16) + r; packed two tiles at a time they are exactly the
 // A fragments of layer 2 in the permuted K order of w2_k_of().  Layer 2 runs
-// on v_mfma_f32_16x16x32_{bf16,fp8}; layer 3 is folded in as sum gelu(z2) w3.
+// on v_mfma_f32_16x16x32_{bf16,fp8} over pairs of n-tiles, so that a lane
+// holds z2 of two hidden units for the same candidate: GELU runs on that f16
+// pair and layer 3 is one v_dot2c_f32_f16 with the matching w3 pair.
 template <int SC>
-__device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const __bf16 *xs, float *sc, const void *w2lds,
-                                          const float *b1s, const float *b2s, const float *w3s, int lane) {
+__device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml, int lane) {
     const int q = lane >> 4, j = lane & 15;
     typedef typename std::conditional<SC == SC_MLP_FP8, uint64_t, bf16x8>::type Frag;
+    const h2 q1 = h2s((_Float16)w.q1), q2 = h2s((_Float16)GELU_Q);
     Frag af[4][8];
 #pragma unroll
     for (int c4 = 0; c4 < 4; ++c4) {
-        const s16x4 bx = *reinterpret_cast<const s16x4 *>(xs + (c4 * 16 + j) * MLP_IN + 4 * q);
+        const s16x4 bx = *reinterpret_cast<const s16x4 *>(ml.xs + (c4 * 16 + j) * MLP_IN + 4 * q);
 #pragma unroll
         for (int mt = 0; mt < 16; ++mt) {
-            f32x4 acc = *reinterpret_cast<const f32x4 *>(b1s + 16 * mt + 4 * q);
-            const bf16x4 wa = w.w1f[mt * 64 + lane];
+            f32x4 acc = *reinterpret_cast<const f32x4 *>(ml.b1 + 16 * mt + 4 * q);
+            const bf16x4 wa = ml.w1f[mt * 64 + lane];
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const s16x4 *>(&wa), bx, acc, 0, 0, 0);
-            float h[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) h[r] = gelu_f(acc[r]);
+            const h2 x01 = pk_rtz(acc[0], acc[1]), x23 = pk_rtz(acc[2], acc[3]);
+            const h2 g01 = x01 * gelu_sig_h2(x01, q1), g23 = x23 * gelu_sig_h2(x23, q1);
             if constexpr (SC == SC_MLP_FP8) {
-                uint32_t pk = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(h[0] * w.h_scale, h[1] * w.h_scale, 0, false);
-                pk = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(h[2] * w.h_scale, h[3] * w.h_scale, (int)pk, true);
-                if (mt & 1) af[c4][mt >> 1] |= (uint64_t)pk << 32;
-                else af[c4][mt >> 1] = (uint64_t)pk;
+                typedef short s16x2 __attribute__((ext_vector_type(2)));
+                s16x2 pk = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(s16x2{0, 0}, g01, 1.0f, false);
+                pk = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(pk, g23, 1.0f, true);
+                const uint64_t u = __builtin_bit_cast(uint32_t, pk);
+                if (mt & 1) af[c4][mt >> 1] |= u << 32;
+                else af[c4][mt >> 1] = u;
             } else {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) af[c4][mt >> 1][(mt & 1) * 4 + r] = (__bf16)h[r];
+                const bf16x2 b01 = __builtin_convertvector(__builtin_convertvector(g01, f32x2), bf16x2);
+                const bf16x2 b23 = __builtin_convertvector(__builtin_convertvector(g23, f32x2), bf16x2);
+                af[c4][mt >> 1][(mt & 1) * 4 + 0] = b01[0];
+                af[c4][mt >> 1][(mt & 1) * 4 + 1] = b01[1];
+                af[c4][mt >> 1][(mt & 1) * 4 + 2] = b23[0];
+                af[c4][mt >> 1][(mt & 1) * 4 + 3] = b23[1];
             }
         }
     }
-    f32x4 part[4];
+    float part[4][4];
 #pragma unroll
-    for (int c4 = 0; c4 < 4; ++c4) part[c4] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int nt = 0; nt < 16; ++nt) {
-        Frag wb[8];
+    for (int c4 = 0; c4 < 4; ++c4)
 #pragma unroll
-        for (int kb = 0; kb < 8; ++kb) wb[kb] = reinterpret_cast<const Frag *>(w2lds)[(kb * 16 + nt) * 64 + lane];
-        const float b2n = b2s[16 * nt + j], w3n = w3s[16 * nt + j];
+        for (int r = 0; r < 4; ++r) part[c4][r] = 0.0f;
+    for (int np = 0; np < 8; ++np) {
+        const float ba = ml.b2[32 * np + j], bb = ml.b2[32 * np + 16 + j];
+        f32x4 ca[4], cb[4];
 #pragma unroll
         for (int c4 = 0; c4 < 4; ++c4) {
-            f32x4 acc = SC == SC_MLP_FP8 ? f32x4{0.f, 0.f, 0.f, 0.f} : f32x4{b2n, b2n, b2n, b2n};
+            ca[c4] = f32x4{ba, ba, ba, ba};
+            cb[c4] = f32x4{bb, bb, bb, bb};
+        }
+        // W2 fragments stream through per k-block: eight independent MFMAs each
 #pragma unroll
-            for (int kb = 0; kb < 8; ++kb) {
-                if constexpr (SC == SC_MLP_FP8)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)af[c4][kb], (long)wb[kb], acc, 0, 0, 0);
-                else
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c4][kb], wb[kb], acc, 0, 0, 0);
-            }
+        for (int kb = 0; kb < 8; ++kb) {
+            const Frag wa = reinterpret_cast<const Frag *>(ml.w2)[(kb * 16 + 2 * np) * 64 + lane];
+            const Frag wb = reinterpret_cast<const Frag *>(ml.w2)[(kb * 16 + 2 * np + 1) * 64 + lane];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float z = SC == SC_MLP_FP8 ? acc[r] * w.descale + b2n : acc[r];
-                part[c4][r] += gelu_f(z) * w3n;
+            for (int c4 = 0; c4 < 4; ++c4) {
+                if constexpr (SC == SC_MLP_FP8) {
+                    ca[c4] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)af[c4][kb], (long)wa, ca[c4], 0, 0, 0);
+                    cb[c4] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)af[c4][kb], (long)wb, cb[c4], 0, 0, 0);
+                } else {
+                    ca[c4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c4][kb], wa, ca[c4], 0, 0, 0);
+                    cb[c4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c4][kb], wb, cb[c4], 0, 0, 0);
+                }
             }
         }
+        const h2 w3p = __builtin_bit_cast(h2, ml.w3h[16 * np + j]);
+#pragma unroll
+        for (int c4 = 0; c4 < 4; ++c4)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float za = SC == SC_MLP_FP8 ? ca[c4][r] * w.descale : ca[c4][r];
+                const float zb = SC == SC_MLP_FP8 ? cb[c4][r] * w.descale : cb[c4][r];
+                const h2 x = pk_rtz(za, zb);
+                part[c4][r] = __builtin_amdgcn_fdot2(x * gelu_sig_h2(x, q2), w3p, part[c4][r], false);
+            }
     }
 #pragma unroll
     for (int c4 = 0; c4 < 4; ++c4)
@@ -479,52 +513,28 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const __bf16 *xs,
             v += __shfl_xor(v, 2);
             v += __shfl_xor(v, 4);
             v += __shfl_xor(v, 8);
-            if (j == 0) sc[c4 * 16 + 4 * q + r] = v + w.b3;
+            if (j == 0) ml.sc[c4 * 16 + 4 * q + r] = v + w.b3;
         }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    return sc[lane];
+    return ml.sc[lane];
 }
 
-struct MlpLds {
-    const void *w2;
-    const float *b1, *b2, *w3;
-    __bf16 *xs;
-    float *sc;
-};
-
-template <bool IMM, bool LDS_ELITES, bool FAST_MUT, int SC, bool DUMP>
-__device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml,
-                                            const float *el, const uint32_t *q, uint32_t head,
-                                            uint32_t count, uint64_t bstart, uint64_t first,
-                                            uint64_t seed, uint32_t stream, float bound,
-                                            int lane) {
-    const bool act = (uint32_t)lane < count;
-    const uint32_t code = act ? q[(head + (uint32_t)lane) & (QCAP - 1)] : 0u;
-    const uint64_t c = bstart + (code & 0xffffu);
-    const uint64_t cidx = first + ((code >> 16) & 0x7ffu);
-    const bool jp = (code >> 28) & 1u;
-    uint64_t s = seed + (c + 1) * GAMMA;
-    const uint32_t ia = fastmod(draw_hi(s), a.ne_magic, a.ne);
-    s += GAMMA;
-    const uint32_t ib = fastmod(draw_hi(s), a.ne_magic, a.ne);
-    s += 2 * GAMMA;  // the immigrant test draw is already known
-    float g[CS_NTOT];
+// Genes [K0, K1) of one candidate.  s0 is the stream state of the first gene
+// draw; gene k uses draws s0 + 2k (crossover bit) and s0 + 2k + 1 (mutation
+// unit), or s0 + k (immigrant unit).
+template <bool IMM, bool FAST_MUT, int K0, int K1>
+__device__ __forceinline__ void synth_genes(float *g, uint64_t s0, const float *el, uint32_t ia, uint32_t ib,
+                                            float sigma, float sig24) {
     if (IMM) {
 #pragma unroll
-        for (int k = 0; k < CS_NTOT; ++k) {
-            g[k] = unit_f(draw_u24(s));
-            s += GAMMA;
-        }
+        for (int k = K0; k < K1; ++k) g[k] = unit_f(draw_u24(s0 + (uint64_t)k * GAMMA));
     } else {
-        const float sigma = a.sigma, sig24 = a.sig24;
 #pragma unroll
-        for (int k = 0; k < CS_NTOT; ++k) {
-            const uint32_t bit = draw_bit(s);
-            s += GAMMA;
-            const uint32_t mu = draw_u24(s);
-            s += GAMMA;
+        for (int k = K0; k < K1; ++k) {
+            const uint32_t bit = draw_bit(s0 + (uint64_t)(2 * k) * GAMMA);
+            const uint32_t mu = draw_u24(s0 + (uint64_t)(2 * k + 1) * GAMMA);
             const uint32_t e = bit ? ib : ia;
             const float par = el[e * CS_NTOT + k];
             if (FAST_MUT) {
@@ -542,14 +552,45 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
             }
         }
     }
-    s += GAMMA;  // the jump test draw is already known
+}
+
+// IMM: 0 normal candidates, 1 immigrants, 2 either, by the uniform imm_rt
+// (one inlined copy of the scorer for both kinds: the MLP variants).
+template <int IMM, bool LDS_ELITES, bool FAST_MUT, int SC, bool DUMP>
+__device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml,
+                                            const float *el, const uint32_t *q, uint32_t head,
+                                            uint32_t count, uint64_t bstart, uint64_t first,
+                                            uint64_t seed, uint32_t stream, float bound,
+                                            int lane, bool imm_rt = false) {
+    const bool imm = IMM == 2 ? imm_rt : IMM == 1;
+    const bool act = (uint32_t)lane < count;
+    const uint32_t code = act ? q[(head + (uint32_t)lane) & (QCAP - 1)] : 0u;
+    const uint64_t c = bstart + (code & 0xffffu);
+    const uint64_t cidx = first + ((code >> 16) & 0x7ffu);
+    const bool jp = (code >> 28) & 1u;
+    uint64_t s = seed + (c + 1) * GAMMA;
+    const uint32_t ia = fastmod(draw_hi(s), a.ne_magic, a.ne);
+    s += GAMMA;
+    const uint32_t ib = fastmod(draw_hi(s), a.ne_magic, a.ne);
+    // the immigrant test draw (one step on) is already known; genes start two steps on
+    const uint64_t s0 = s + 2 * GAMMA;
+    // The MLP reads genes 0..15 only: the whole genome is synthesized again
+    // after scoring, and only in batches that emit (or dump) a candidate, so
+    // no gene stays live across the network.
+    constexpr int KH = SC == SC_QUAD ? CS_NTOT : MLP_IN;
+    float g[CS_NTOT];
+    if (imm) synth_genes<true, FAST_MUT, 0, KH>(g, s0, el, ia, ib, a.sigma, a.sig24);
+    else synth_genes<false, FAST_MUT, 0, KH>(g, s0, el, ia, ib, a.sigma, a.sig24);
+    // jump test draw at s0 + ngene draws (already known), then index and value
+    const uint64_t sj = s0 + (uint64_t)(imm ? CS_NTOT : 2 * CS_NTOT) * GAMMA;
+    uint32_t gi = CS_NTOT;
+    float jval = 0.0f;
     if (jp) {
-        const uint32_t gi = draw_hi(s) % (uint32_t)CS_NTOT;
-        s += GAMMA;
-        const float val = unit_f(draw_u24(s));
-#pragma unroll
-        for (int k = 0; k < CS_NTOT; ++k) g[k] = (k == (int)gi) ? val : g[k];
+        gi = draw_hi(sj + GAMMA) % (uint32_t)CS_NTOT;
+        jval = unit_f(draw_u24(sj + 2 * GAMMA));
     }
+#pragma unroll
+    for (int k = 0; k < KH; ++k) g[k] = (k == (int)gi) ? jval : g[k];
     float score;
     if constexpr (SC != SC_QUAD) {
         bf16x8 x0, x1;
@@ -563,7 +604,7 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        score = mlp_wave<SC>(a.mlp, ml.xs, ml.sc, ml.w2, ml.b1, ml.b2, ml.w3, lane);
+        score = mlp_wave<SC>(a.mlp, ml, lane);
     } else {
     // Model.score: lanes j = 0..15 of acc + sum, ordered sum, then the
     // linear tail over genes 16..28 (src/screen.zig:97-112).  The weights are
@@ -588,15 +629,22 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
     for (int i = 16; i < CS_NTOT; ++i) r = r + md->lin[i] * g[i];
     score = S + r;
     }
+    const bool pass = act && !(score <= bound);
+    const uint64_t pm = __ballot(pass);
+    if constexpr (KH < CS_NTOT) {
+        if (DUMP || pm) {
+            if (imm) synth_genes<true, FAST_MUT, 0, CS_NTOT>(g, s0, el, ia, ib, a.sigma, a.sig24);
+            else synth_genes<false, FAST_MUT, 0, CS_NTOT>(g, s0, el, ia, ib, a.sigma, a.sig24);
+#pragma unroll
+            for (int k = 0; k < CS_NTOT; ++k) g[k] = (k == (int)gi) ? jval : g[k];
+        }
+    }
     if (DUMP && act) {
         const uint64_t o = a.streams[stream].cand_base + cidx;
 #pragma unroll
         for (int k = 0; k < CS_NTOT; ++k) a.dump_genes[o * CS_NTOT + k] = g[k];
         a.dump_scores[o] = score;
     }
-
-    const bool pass = act && !(score <= bound);
-    const uint64_t pm = __ballot(pass);
     if (pm) {
         uint32_t basei = 0;
         if (lane == 0) basei = atomicAdd(a.rec_count, (uint32_t)__popcll(pm));
@@ -620,10 +668,11 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
 }
 
 template <bool LDS_ELITES, bool FAST_MUT, int SC, bool DUMP = false>
-__global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
+__global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
+    constexpr int NW = score_waves<SC>();
     extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
-    uint32_t *qbase = smem;  // 4 waves x 2 rings x QCAP
-    float *el_lds = reinterpret_cast<float *>(smem + 4 * 2 * QCAP);
+    uint32_t *qbase = smem;  // NW waves x 2 rings x QCAP
+    float *el_lds = reinterpret_cast<float *>(smem + NW * 2 * QCAP);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const float *el = a.elites;
     if (LDS_ELITES) {
@@ -633,32 +682,43 @@ __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
     }
     MlpLds ml{};
     if constexpr (SC != SC_QUAD) {
-        // after the rings: W2 fragments, then b1 | b2 | w3, then per-wave xs and scores
-        char *base = reinterpret_cast<char *>(smem + 4 * 2 * QCAP);
+        // after the rings: W2 fragments, b1 | b2 | w3 pairs, then per wave the
+        // genes (the scores reuse that buffer once layer 1 has read it).  The
+        // FP8 screen carries H1 scaled by the power of two h_scale (exact on
+        // W1 and b1) and Z2 by h_scale w_scale.
+        char *base = reinterpret_cast<char *>(smem + NW * 2 * QCAP);
         const size_t w2b = SC == SC_MLP_FP8 ? MLP_W2_FP8_BYTES : MLP_W2_BF16_BYTES;
         const uint4 *src = SC == SC_MLP_FP8 ? reinterpret_cast<const uint4 *>(a.mlp.w2f8)
                                             : reinterpret_cast<const uint4 *>(a.mlp.w2f);
         uint4 *dst = reinterpret_cast<uint4 *>(base);
         for (uint32_t i = threadIdx.x; i < w2b / 16; i += blockDim.x) dst[i] = src[i];
+        const float hs = SC == SC_MLP_FP8 ? a.mlp.h_scale : 1.0f;
+        const float zs = SC == SC_MLP_FP8 ? a.mlp.h_scale * a.mlp.w_scale : 1.0f;
         float *vec = reinterpret_cast<float *>(base + w2b);
+        uint32_t *w3h = reinterpret_cast<uint32_t *>(vec + 2 * MLP_H);
         for (uint32_t i = threadIdx.x; i < MLP_H; i += blockDim.x) {
-            vec[i] = a.mlp.b1[i];
-            vec[MLP_H + i] = a.mlp.b2[i];
-            vec[2 * MLP_H + i] = a.mlp.w3[i];
+            vec[i] = a.mlp.b1[i] * hs;
+            vec[MLP_H + i] = a.mlp.b2[i] * zs;
+        }
+        for (uint32_t i = threadIdx.x; i < MLP_H / 2; i += blockDim.x) {
+            const uint32_t p = i >> 4, jj = i & 15;
+            const h2 pr{(_Float16)a.mlp.w3[32 * p + jj], (_Float16)a.mlp.w3[32 * p + 16 + jj]};
+            w3h[i] = __builtin_bit_cast(uint32_t, pr);
         }
         char *wv = base + w2b + MLP_VEC_BYTES + (size_t)wave * MLP_WAVE_BYTES;
         ml.w2 = base;
+        ml.w1f = SC == SC_MLP_FP8 ? a.mlp.w1fh : a.mlp.w1f;
         ml.b1 = vec;
         ml.b2 = vec + MLP_H;
-        ml.w3 = vec + 2 * MLP_H;
+        ml.w3h = w3h;
         ml.xs = reinterpret_cast<__bf16 *>(wv);
-        ml.sc = reinterpret_cast<float *>(wv + 64 * MLP_IN * 2);
+        ml.sc = reinterpret_cast<float *>(wv);
         __syncthreads();
     }
     uint32_t *qN = qbase + wave * 2 * QCAP;
     uint32_t *qI = qN + QCAP;
     unsigned long long screened = 0;
-    for (uint64_t fb = (uint64_t)blockIdx.x * 4 + wave; fb < a.nflat; fb += (uint64_t)gridDim.x * 4) {
+    for (uint64_t fb = (uint64_t)blockIdx.x * NW + wave; fb < a.nflat; fb += (uint64_t)gridDim.x * NW) {
         const int t = find_seg(a.seg_first, a.T, fb);
         const LaunchSeg sg = a.segs[t];
         const StreamInfo st = a.streams[t];
@@ -671,7 +731,8 @@ __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
         uint64_t idx = first;
         uint32_t headN = 0, tailN = 0, headI = 0, tailI = 0;
         const uint64_t lt = (1ull << lane) - 1ull;
-        while (c < bend && idx < st.my_lambda) {
+        // one step of the chain walk: up to 64 candidate starts into the rings
+        auto enqueue = [&]() {
             // speculate: lane j starts at c + 62 j (no immigrant, no jump before it)
             const uint64_t cj = c + 62ull * (uint64_t)lane;
             const bool im = draw_u24(st.seed + (cj + 3) * GAMMA) < a.immT;
@@ -697,29 +758,60 @@ __global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            while (tailN - headN >= 64) {
-                score_batch<false, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qN, headN, 64, bstart, first, st.seed, (uint32_t)t,
-                                               sg.bound, lane);
-                headN += 64;
-                screened += 64;
+        };
+        if constexpr (SC == SC_QUAD) {
+            while (c < bend && idx < st.my_lambda) {
+                enqueue();
+                while (tailN - headN >= 64) {
+                    score_batch<0, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qN, headN, 64, bstart, first, st.seed,
+                                                                   (uint32_t)t, sg.bound, lane);
+                    headN += 64;
+                    screened += 64;
+                }
+                if (tailI - headI >= 64) {
+                    score_batch<1, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qI, headI, 64, bstart, first, st.seed,
+                                                                   (uint32_t)t, sg.bound, lane);
+                    headI += 64;
+                    screened += 64;
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-            if (tailI - headI >= 64) {
-                score_batch<true, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qI, headI, 64, bstart, first, st.seed, (uint32_t)t,
-                                              sg.bound, lane);
-                headI += 64;
-                screened += 64;
+            if (tailN > headN) {
+                score_batch<0, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qN, headN, tailN - headN, bstart, first,
+                                                               st.seed, (uint32_t)t, sg.bound, lane);
+                screened += tailN - headN;
             }
-            __builtin_amdgcn_wave_barrier();
-        }
-        if (tailN > headN) {
-            score_batch<false, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qN, headN, tailN - headN, bstart, first, st.seed,
-                                           (uint32_t)t, sg.bound, lane);
-            screened += tailN - headN;
-        }
-        if (tailI > headI) {
-            score_batch<true, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qI, headI, tailI - headI, bstart, first, st.seed,
-                                          (uint32_t)t, sg.bound, lane);
-            screened += tailI - headI;
+            if (tailI > headI) {
+                score_batch<1, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qI, headI, tailI - headI, bstart, first,
+                                                               st.seed, (uint32_t)t, sg.bound, lane);
+                screened += tailI - headI;
+            }
+        } else {
+            // One call site for every batch (full or final, either kind):
+            // the rings are refilled only when both hold fewer than 64, so
+            // they never exceed 127 < QCAP entries.
+            bool more = c < bend && idx < st.my_lambda;
+            for (;;) {
+                if (more && tailN - headN < 64 && tailI - headI < 64) {
+                    enqueue();
+                    more = c < bend && idx < st.my_lambda;
+                    continue;
+                }
+                bool im;
+                uint32_t cnt;
+                if (tailN - headN >= 64) { im = false; cnt = 64; }
+                else if (tailI - headI >= 64) { im = true; cnt = 64; }
+                else if (tailN > headN) { im = false; cnt = tailN - headN; }
+                else if (tailI > headI) { im = true; cnt = tailI - headI; }
+                else break;
+                score_batch<2, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, im ? qI : qN, im ? headI : headN, cnt,
+                                                               bstart, first, st.seed, (uint32_t)t, sg.bound, lane,
+                                                               im);
+                if (im) headI += cnt;
+                else headN += cnt;
+                screened += cnt;
+                __builtin_amdgcn_wave_barrier();
+            }
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -792,7 +884,8 @@ static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std
     a.segs = (const LaunchSeg *)ctx->segs.p;
     a.seg_first = (const uint64_t *)ctx->seg_first.p;
     a.nflat = nflat;
-    uint64_t grid = (nflat + 3) / 4;
+    const unsigned nw = h.sc == SC_QUAD ? score_waves<SC_QUAD>() : score_waves<SC_MLP_BF16>();
+    uint64_t grid = (nflat + nw - 1) / nw;
     if (grid > (1u << 20)) grid = 1u << 20;
     HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
     if (h.sc != SC_QUAD) {
@@ -804,7 +897,7 @@ static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std
         if (h.smem > 64 * 1024)
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)h.smem);
-        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), h.smem, ctx->stream, a);
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * nw), h.smem, ctx->stream, a);
     };
     if (a.dump_genes) {  // evaluation runs: every candidate's genes and score
         if (h.sc == SC_MLP_BF16) go(k_score<false, false, SC_MLP_BF16, true>);
@@ -1143,6 +1236,7 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         HIPCHK(hipStreamSynchronize(ctx->stream));
         MlpScreenW &mw = h.sa.mlp;
         mw.w1f = (const bf16x4 *)ml->w1f.p;
+        mw.w1fh = (const bf16x4 *)ml->w1fh.p;
         mw.w2f = (const bf16x8 *)ml->w2f.p;
         mw.w2f8 = (const uint64_t *)ml->w2f8.p;
         mw.b1 = pp + P_B1;
@@ -1152,6 +1246,7 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         mw.h_scale = ml->h_scale;
         mw.w_scale = ml->w_scale;
         mw.descale = 1.0f / (ml->h_scale * ml->w_scale);
+        mw.q1 = sc == SC_MLP_FP8 ? GELU_Q / ml->h_scale : GELU_Q;
     }
     ScoreArgs &a = h.sa;
     a.streams = (const StreamInfo *)ctx->streams.p;
@@ -1180,9 +1275,13 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     a.cand_count = (unsigned long long *)((char *)ctx->counters.p + 8);
     const size_t el_bytes = (size_t)ne * CS_NTOT * 4;
     h.lds_elites = sc == SC_QUAD && el_bytes <= 48 * 1024 && h.sa.dump_genes == nullptr;
-    h.smem = 4 * 2 * QCAP * 4 + (h.lds_elites ? el_bytes : 0);
-    if (sc != SC_QUAD)
-        h.smem += (sc == SC_MLP_FP8 ? MLP_W2_FP8_BYTES : MLP_W2_BF16_BYTES) + MLP_VEC_BYTES + 4 * MLP_WAVE_BYTES;
+    if (sc == SC_QUAD) {
+        h.smem = score_waves<SC_QUAD>() * 2 * QCAP * 4 + (h.lds_elites ? el_bytes : 0);
+    } else {
+        const size_t nw = score_waves<SC_MLP_BF16>();
+        h.smem = nw * 2 * QCAP * 4 + (sc == SC_MLP_FP8 ? MLP_W2_FP8_BYTES : MLP_W2_BF16_BYTES) + MLP_VEC_BYTES +
+                 nw * MLP_WAVE_BYTES;
+    }
     {
         std::vector<uint64_t> lo(T), hi(T);
         uint64_t maxnb = 0;

@@ -160,7 +160,8 @@ int cs_mlp_create(cs_ctx *ctx, uint64_t seed, cs_mlp **out);
 int cs_mlp_destroy(cs_mlp *mlp);
 int cs_mlp_get_params(cs_mlp *mlp, float *host_params);
 int cs_mlp_set_params(cs_mlp *mlp, const float *host_params);
-/* FP8 screen scales: H1 and W2 are multiplied by these before E4M3 rounding */
+/* FP8 screen scales: H1 and W2 are multiplied by these before E4M3 rounding.
+ * Both must be powers of two in [2^-15, 2^15] (else CS_E_BADARGS). */
 int cs_mlp_set_fp8_scales(cs_mlp *mlp, float h_scale, float w_scale);
 /* out[n] (device) = MLP(genes); *h1max (optional) = max |H1| for FP8 calibration */
 int cs_mlp_forward(cs_mlp *mlp, const float *genes, int ldg, uint64_t n, float *out, float *h1max);

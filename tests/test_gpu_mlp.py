@@ -258,3 +258,24 @@ def test_context_outlives_its_mlps():
     d.close()
     assert m._h is None and d._h is None
     del p
+
+
+@pytest.mark.parametrize("scorer", [1, 2])
+def test_mlp_screen_survivor_genomes(dev, trained, scorer):
+    """Genes 16..28 are synthesized only in batches that emit: the survivors of
+    a plain screen (genomes included) equal those of the dumping screen, whose
+    genes the oracle's synthesis confirms."""
+    from csfunnel import screen as scr
+    from csfunnel.mlp import screen_ex
+    m = trained[0]
+    g = os.path.join(REPO, "tests", "golden")
+    el = scr.load_elites(os.path.join(g, "c1_elites.txt"))
+    cfg = scr.Cfg(1 << 21, 256, 9, 4, float(np.float32(0.12)), float(np.float32(0.05)), float(np.float32(0.05)))
+    print(f"case survivor genomes scorer={scorer} lam=2^21 keep=256", flush=True)
+    a, n, dg, ds = screen_ex(dev, None, el, cfg, scorer, m, dump=True)
+    b, n2, _, _ = screen_ex(dev, None, el, cfg, scorer, m)
+    assert n == n2 == 1 << 21
+    assert np.array_equal(a, b)
+    per = (1 << 21) // 4
+    rows = a["stream"].astype(np.int64) * per + a["index"].astype(np.int64)
+    assert np.array_equal(a["g"], dg.cpu().numpy()[rows])
