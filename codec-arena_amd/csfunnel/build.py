@@ -19,6 +19,7 @@ PKG_ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(PKG_ROOT, "csrc")
 REPO = os.path.dirname(PKG_ROOT)
 LIB = os.path.join(HERE, "libcsfunnel.so")
+CODEC_LIB = os.path.join(HERE, "libcscodec.so")
 
 SOURCES = ["screen_device.hip", "ridge_device.hip", "mlp_device.hip", "funnel_host.cpp"]
 ARCH = "gfx950"
@@ -47,7 +48,24 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError(f"hipcc failed (rc={r.returncode})")
     os.replace(LIB + ".tmp", LIB)
+    build_codec(verbose)
     return LIB
+
+
+def build_codec(verbose: bool = False) -> str:
+    """The codec label source is plain host C: built with gcc for a baseline
+    x86-64-v3 target (it runs on whichever host the package is shipped to)."""
+    cmd = [os.environ.get("CC", "gcc"), "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math",
+           "-fPIC", "-shared", "-std=gnu11", "-Wall", "-Wextra", "-I", os.path.join(REPO, "include"),
+           "-o", CODEC_LIB + ".tmp", os.path.join(CSRC, "codec_eval.c"), "-lm", "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"gcc failed (rc={r.returncode})")
+    os.replace(CODEC_LIB + ".tmp", CODEC_LIB)
+    return CODEC_LIB
 
 
 if __name__ == "__main__":

@@ -1,0 +1,19 @@
+"""Writes tests/golden/halloffame_kats.json: the genome and published metrics
+of every record of codec-arena's halloffame/pareto.json and
+halloffame/champion.json (data files of the reference snapshot), the known
+answers the codec label source is checked against.
+
+    python tests/golden/make_halloffame_kats.py /root/reference
+"""
+import json
+import os
+import sys
+
+ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+recs = json.load(open(os.path.join(ref, "halloffame", "pareto.json")))
+champ = json.load(open(os.path.join(ref, "halloffame", "champion.json")))
+out = {"source": "halloffame/pareto.json, halloffame/champion.json",
+       "records": [{"id": r["id"], "g": r["g"], "metrics": r["metrics"]} for r in recs],
+       "champion": {"id": champ["id"], "g": champ["g"], "metrics": champ["metrics"]}}
+with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "halloffame_kats.json"), "w") as f:
+    json.dump(out, f, indent=1)

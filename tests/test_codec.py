@@ -1,0 +1,223 @@
+"""The codec label source (CPU): corpus, encoder/decoder, metrics.
+
+Pinned by the reference's own data: the corpus against corpus/manifest.json,
+and every hall-of-fame record whose tools are restated against its published
+PSNR, bpp, dec_mops, chroma PSNR, MS-SSIM and MS-SSIM dB (halloffame/*.json,
+measured by the reference on the five legacy clips of tools/gen_corpus.py).
+Beside them: the closed-loop invariant of src/codec.zig:2833-2895, the range
+coder round trip of src/codec.zig:3001-3094, the MS-SSIM and PSNR properties of
+src/msssim.zig:186-226 and src/yuv.zig:82-85, and an FP64 MS-SSIM written from
+Wang et al. 2003.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+LEGACY5 = ("drift", "grid", "storm", "quake", "grain")
+PRINTED = {"psnr": 2, "bpp": 4, "dec_mops": 2, "psnr_u": 2, "psnr_v": 2, "msssim": 5, "msssim_db": 4}
+
+
+@pytest.fixture(scope="module")
+def cc():
+    from csfunnel import codec
+    codec.lib()
+    return codec
+
+
+@pytest.fixture(scope="module")
+def clips(tmp_path_factory):
+    from csfunnel import corpus
+    d = str(tmp_path_factory.mktemp("corpus"))
+    corpus.generate(d)
+    return d
+
+
+def test_corpus_matches_manifest(clips):
+    from csfunnel import corpus
+    for n in corpus.NAMES:
+        with open(os.path.join(clips, n + ".yuv"), "rb") as f:
+            data = f.read()
+        assert len(data) == corpus.FRAMES * corpus.W * corpus.H * 3 // 2
+        assert hashlib.sha256(data).hexdigest() == corpus.MANIFEST[n]
+    assert open(os.path.join(clips, "corpus.txt")).read().splitlines()[0] == "drift 320 192 24 24"
+
+
+def _kat_genome(cc, r):
+    # the oracle reads genomes from pop lines written with "%.4f" (brain/evolve.py:104-106)
+    return cc.genome([float("%.4f" % x) for x in r["g"]], r["id"])
+
+
+def test_halloffame_records_reproduce(cc, clips):
+    kats = json.load(open(os.path.join(GOLDEN, "halloffame_kats.json")))
+    recs = kats["records"] + [kats["champion"]]
+    ev = cc.Evaluator(names=LEGACY5, corpus_dir=clips, msssim=True)
+    gs = [_kat_genome(cc, r) for r in recs]
+    keep = [i for i, g in enumerate(gs) if cc.supported(g)]
+    assert len(keep) == len(recs) - 1  # one record switches CDEF on
+    ms = ev.evaluate([gs[i] for i in keep])
+    checked = 0
+    for i, m in zip(keep, ms):
+        ref = recs[i]["metrics"]
+        for k, d in PRINTED.items():
+            if k in ref:
+                assert round(m[k], d) == pytest.approx(ref[k], abs=0.51 * 10 ** -d), (recs[i]["id"], k, m[k], ref[k])
+                checked += 1
+    assert checked >= 100
+
+
+def test_unsupported_tools_rejected(cc):
+    base = [0.5] * 16 + [0.0] * 7 + [0.5] * 6
+    for k, v in ((16, 0.25), (19, 0.51), (20, 0.51), (21, 0.51), (22, 0.51)):
+        g = list(base)
+        g[k] = v
+        assert not cc.supported(cc.genome(g))
+    for k, v in ((16, 0.2499), (17, 1.0), (18, 1.0), (19, 0.5), (20, 0.5), (21, 0.5), (22, 0.5)):
+        g = list(base)
+        g[k] = v
+        assert cc.supported(cc.genome(g))
+    g = list(base)
+    g[20] = 0.9
+    with pytest.raises(cc.CodecError):
+        cc.eval_clip(bytes(320 * 192 * 3 // 2 * 2), cc.genome(g), frames=2)
+
+
+def test_closed_loop_holds_on_the_range_coder_stream(cc, clips):
+    """Decoder output == encoder recon, byte for byte, with the adaptive range
+    coder on, half-pel, long GOPs, intra refresh, adaptive q and deblocking."""
+    from csfunnel import corpus
+    data = corpus.load(clips)
+    rng = np.random.default_rng(11)
+    for i in range(6):
+        g = list(np.round(rng.random(16), 4)) + [0.0, float(rng.random()), 0.8, 0.0, 0.0, 0.0, 0.0] + \
+            list(np.round(rng.random(6), 4))
+        g[12] = 0.9 if i % 2 == 0 else 0.1
+        gn = cc.genome(g, i)
+        for n in ("quake", "zoom"):
+            st = cc.eval_clip(data[n], gn, flags=cc.FAST_SAD | cc.CLOSED_LOOP)
+            assert st.closed_loop == 1 and st.idct_calls == 2 * st.dct_calls
+
+
+def test_varint_stream_reproduces_the_reference_decode(cc, clips):
+    """The byte-varint decoder reads a negative x back as x + 1
+    (src/codec.zig:215-220), so that stream's decoder drifts from the encoder."""
+    L = cc.lib()
+    for v in (0, 1, 2, 63, 64, 1 << 40):
+        assert L.csc_svarint_roundtrip(v) == v
+    for v in (-1, -2, -3, -64, -65, -(1 << 40)):
+        assert L.csc_svarint_roundtrip(v) == v + 1
+    from csfunnel import corpus
+    data = corpus.load(clips, ("drift",))["drift"]
+    g = [0.3] * 16
+    st = cc.eval_clip(data, cc.genome(g))
+    assert st.closed_loop == 0
+    with pytest.raises(cc.CodecError):
+        cc.eval_clip(data, cc.genome(g), flags=cc.FAST_SAD | cc.CLOSED_LOOP)
+    g_on = cc.genome(g[:16] + [0.0, 0.0, 0.9, 0.0, 0.0, 0.0, 0.0])
+    on = cc.eval_clip(data, g_on, flags=cc.FAST_SAD | cc.CLOSED_LOOP)
+    assert on.closed_loop == 1 and on.bytes < st.bytes
+
+
+def test_range_coder_roundtrip(cc):
+    import ctypes as C
+    L = cc.lib()
+    rng = np.random.default_rng(5)
+    for n, p1 in ((0, 0.5), (1, 0.5), (5000, 0.5), (20000, 0.02), (20000, 0.98)):
+        bits = (rng.random(n) < p1).astype(np.uint8)
+        ctx = rng.integers(0, 64, n).astype(np.uint8)
+        buf = C.create_string_buffer(n + 64)
+        back = C.create_string_buffer(max(1, n))
+        m = L.csc_rc_roundtrip(bits.tobytes(), ctx.tobytes(), n, buf, n + 64, back)
+        assert m >= 5
+        assert np.frombuffer(back.raw[:n], dtype=np.uint8).tolist() == bits.tolist()
+        if n == 0:
+            assert m == 5
+        if n == 20000:
+            # adaptive probabilities: skewed streams cost well under a bit each
+            h = -(p1 * np.log2(p1) + (1 - p1) * np.log2(1 - p1))
+            assert m * 8 < n * (h + 0.1)
+
+
+def _msssim_f64(a, b):
+    """MS-SSIM (Wang, Simoncelli, Bovik 2003) in FP64: valid-mode 11-tap
+    gaussian sigma 1.5, 2x2 mean pooling, weights (.0448 .2856 .3001 .2363
+    .1333), C1 = (0.01 L)^2, C2 = (0.03 L)^2."""
+    x = np.arange(11) - 5.0
+    k = np.exp(-x * x / 4.5)
+    k /= k.sum()
+
+    def filt(im):
+        h, w = im.shape
+        t = sum(k[i] * im[:, i:i + w - 10] for i in range(11))
+        return sum(k[i] * t[i:i + h - 10, :] for i in range(11))
+
+    W = [0.0448, 0.2856, 0.3001, 0.2363, 0.1333]
+    X, Y = a.astype(np.float64), b.astype(np.float64)
+    out = 0.0
+    for s in range(5):
+        mx, my = filt(X), filt(Y)
+        vx, vy, vxy = filt(X * X) - mx * mx, filt(Y * Y) - my * my, filt(X * Y) - mx * my
+        cs = ((2 * vxy + 58.5225) / (vx + vy + 58.5225)).mean()
+        if s == 4:
+            out += W[s] * np.log(((2 * mx * my + 6.5025) / (mx * mx + my * my + 6.5025)).mean())
+        out += W[s] * np.log(cs)
+        if s < 4:
+            h, w = X.shape
+            X = (X[0:h // 2 * 2:2, 0:w // 2 * 2:2] + X[0:h // 2 * 2:2, 1:w // 2 * 2:2] + X[1:h // 2 * 2:2, 0:w // 2 * 2:2]
+                 + X[1:h // 2 * 2:2, 1:w // 2 * 2:2]) * 0.25
+            Y = (Y[0:h // 2 * 2:2, 0:w // 2 * 2:2] + Y[0:h // 2 * 2:2, 1:w // 2 * 2:2] + Y[1:h // 2 * 2:2, 0:w // 2 * 2:2]
+                 + Y[1:h // 2 * 2:2, 1:w // 2 * 2:2]) * 0.25
+    return float(np.exp(out))
+
+
+def _ms(cc, a, b):
+    import ctypes as C
+    h, w = a.shape
+    v = C.c_float()
+    assert cc.lib().csc_plane_msssim(a.astype(np.uint8).tobytes(), b.astype(np.uint8).tobytes(), w, h, C.byref(v)) == 0
+    return v.value
+
+
+def test_msssim_properties_and_fp64(cc):
+    i = np.arange(64 * 64)
+    buf = ((i * 7) % 256).reshape(64, 64)
+    assert _ms(cc, buf, buf) > 0.9999
+    base = ((i * 3 + (i // 64) * 5) % 256).reshape(64, 64)
+    rng = np.random.default_rng(0xC0DEC)
+    mild = np.minimum(255, np.maximum(0, base + rng.integers(-4, 5, base.shape)))
+    heavy = np.minimum(255, np.maximum(0, base + rng.integers(-40, 41, base.shape)))
+    m, hv = _ms(cc, base, mild), _ms(cc, base, heavy)
+    assert m > hv and 0.05 < hv < 0.999
+    a = ((np.arange(32 * 32) * 11) % 200 + 20).reshape(32, 32)
+    v = _ms(cc, a, a + 10)
+    assert 0.90 < v < 1.0
+    for (h, w) in ((192, 320), (176, 176)):
+        x = rng.integers(0, 256, (h, w))
+        y = np.clip(x + rng.integers(-20, 21, (h, w)), 0, 255)
+        assert _ms(cc, x, y) == pytest.approx(_msssim_f64(x, y), abs=2e-5)
+
+
+def test_psnr_plane(cc):
+    a = np.arange(1000, dtype=np.uint8)
+    assert cc.lib().csc_psnr_plane(a.tobytes(), a.tobytes(), 1000) == 99.0
+    b = (a.astype(np.int64) + np.where(np.arange(1000) % 2 == 0, 3, -3)).clip(0, 255).astype(np.uint8)
+    mse = ((a.astype(np.float64) - b) ** 2).mean()
+    assert cc.lib().csc_psnr_plane(a.tobytes(), b.tobytes(), 1000) == pytest.approx(
+        10 * np.log10(255.0 ** 2 / mse), abs=1e-5)
+
+
+def test_codec_library_exports_every_declared_symbol(cc):
+    import re
+    import ctypes as C
+    txt = open(os.path.join(os.path.dirname(GOLDEN), "..", "include", "cscodec.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    names = sorted(set(re.findall(r"\b(csc_[a-z0-9_]+)\s*\(", txt)))
+    assert len(names) == 6
+    L = cc.lib()
+    for n in names:
+        assert hasattr(L, n) and n in cc.SIGNATURES, n
+    assert C.sizeof(cc.ClipStats) == 8 + 6 * 4 + 8 + 8 + 4 + 4
