@@ -70,6 +70,19 @@ def test_halloffame_records_reproduce(cc, clips):
     assert checked >= 100
 
 
+def test_leaderboard_bot_reproduces_on_the_full_corpus(cc, clips):
+    """Bot 13 of pop/seed.txt read through the reference parser (its 34-token
+    line loses genes 16-21, so the entropy gene is 0): the byte-varint stream
+    with its drifting decode, on all ten clips, gives the LEADERBOARD.md row."""
+    from csfunnel import screen as scr
+    kat = json.load(open(os.path.join(GOLDEN, "halloffame_kats.json")))["leaderboard"]
+    g = scr.parse_line(kat["pop_line"])
+    assert list(g.g)[16:22] == [0.0] * 6 and cc.supported(g)
+    m = cc.Evaluator(corpus_dir=clips, msssim=True).metrics(g)
+    for k, d in (("psnr", 2), ("bpp", 4), ("msssim", 4)):
+        assert round(m[k], d) == pytest.approx(kat["metrics"][k], abs=0.51 * 10 ** -d), (k, m[k])
+
+
 def test_unsupported_tools_rejected(cc):
     base = [0.5] * 16 + [0.0] * 7 + [0.5] * 6
     for k, v in ((16, 0.25), (19, 0.51), (20, 0.51), (21, 0.51), (22, 0.51)):
