@@ -50,6 +50,12 @@ SIGNATURES = {
     "csc_plane_msssim": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
     "csc_psnr_plane": (C.c_float, [C.c_char_p, C.c_char_p, C.c_uint64]),
     "csc_svarint_roundtrip": (C.c_int64, [C.c_int64]),
+    "csc_intra_predict": (C.c_int, [C.c_int, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int,
+                                    C.c_char_p]),
+    "csc_interp_q": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_char_p]),
+    "csc_deblock_pair": (C.c_int, [C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.c_uint8, C.c_uint8, C.c_float,
+                                   C.c_float]),
     "csc_rc_roundtrip": (C.c_int64, [C.c_char_p, C.c_char_p, C.c_uint64, C.c_char_p, C.c_uint64, C.c_char_p]),
 }
 

@@ -11,8 +11,12 @@
  *   adaptive range coder, contexts ... src/codec.zig:239-525
  *   quantizer, levelQ ................ src/codec.zig:752-775, 1446-1483
  *   bilinear half-pel MC ............. src/codec.zig:788-812
+ *   quarter-pel MC, half planes ...... src/codec.zig:813-1006
+ *   intra prediction, mode sets ...... src/codec.zig:1009-1256
  *   deblocking ....................... src/codec.zig:1258-1310
- *   frame / plane encode ............. src/codec.zig:1505-2237 (8x8 legacy syntax)
+ *   CDEF-lite ........................ src/codec.zig:1320-1400
+ *   frame / plane encode ............. src/codec.zig:1505-2237 (8x8 syntax; the
+ *                                      16x16 superblock layer is not restated)
  *   SAD .............................. src/codec.zig:2239-2261
  *   frame / plane decode ............. src/codec.zig:2263-2535
  *   evalClip ......................... src/codec.zig:2537-2670
@@ -76,6 +80,10 @@ typedef struct pheno {
     float gain[6];
     float deblock;
     int entropy;
+    float intra_gene; /* g[16]: intra mode set tier */
+    int cfl;          /* g[19] > 0.5: chroma-from-luma modes join the chroma set */
+    int cdef;         /* g[20] > 0.5: CDEF-lite after deblocking */
+    int qpel;         /* g[21] > 0.5: quarter-pel MC supersedes the bilinear half-pel path */
 } pheno;
 
 static void phenotype(const cs_genome* g, pheno* p) {
@@ -98,12 +106,15 @@ static void phenotype(const cs_genome* g, pheno* p) {
     for (int i = 0; i < 6; i++) p->gain[i] = (clamp01(g->reg[i]) - 0.5f) * 0.5f;
     p->deblock = clamp01(g->g[17]);
     p->entropy = clamp01(g->g[18]) > 0.5f;
+    p->intra_gene = clamp01(g->g[16]);
+    p->cfl = clamp01(g->g[19]) > 0.5f;
+    p->cdef = clamp01(g->g[20]) > 0.5f;
+    p->qpel = g->g[21] > 0.5f;
 }
 
 int csc_genome_supported(const cs_genome* g) {
     if (!g) return 0;
-    return clamp01(g->g[16]) < 0.25f && clamp01(g->g[19]) <= 0.5f && clamp01(g->g[20]) <= 0.5f &&
-           !(g->g[21] > 0.5f) && !(g->g[22] > 0.5f);
+    return !(g->g[22] > 0.5f);
 }
 
 typedef struct regctx {
@@ -466,8 +477,8 @@ static void bw_mv(blockw* w, int32_t dx, int32_t dy) {
     }
 }
 
-static void bw_subpel(blockw* w, uint32_t code) {
-    if (w->ac) rc_ubits(&w->e, CX_SP_B0, 4, code);
+static void bw_subpel(blockw* w, uint32_t code, int bits) {
+    if (w->ac) rc_ubits(&w->e, CX_SP_B0, bits, code);
     else put_u8(w->out, (uint8_t)code);
 }
 
@@ -553,8 +564,8 @@ static int br_mv(blockr* r, int32_t* dx, int32_t* dy) {
     return 0;
 }
 
-static uint32_t br_subpel(blockr* r) {
-    if (r->ac) return rc_dubits(&r->d, CX_SP_B0, 4);
+static uint32_t br_subpel(blockr* r, int bits) {
+    if (r->ac) return rc_dubits(&r->d, CX_SP_B0, bits);
     return get_u8(r->in);
 }
 
@@ -622,6 +633,282 @@ static void interp_block(const uint8_t* prev, int pw, int ph, int ox, int oy, in
     }
 }
 
+/* ---- quarter-pel MC (g[21]): 4-tap (-1, 9, 9, -1)/16 halves, clamped to
+ * [0, 255], bilinear quarters; the half planes are built once per reference
+ * plane and read by both sides (src/codec.zig:828-1006) */
+
+static inline int half_t4(int p0, int p1, int p2, int p3) {
+    const int v = (-p0 + 9 * p1 + 9 * p2 - p3 + 8) / 16; /* truncating division */
+    return v < 0 ? 0 : (v > 255 ? 255 : v);
+}
+
+static inline int at_clamp(const uint8_t* pl, int pw, int ph, int x, int y) {
+    return pl[(size_t)clampi(y, 0, ph - 1) * pw + clampi(x, 0, pw - 1)];
+}
+
+static inline int half_h(const uint8_t* pl, int pw, int ph, int x, int y) {
+    return half_t4(at_clamp(pl, pw, ph, x - 1, y), at_clamp(pl, pw, ph, x, y), at_clamp(pl, pw, ph, x + 1, y),
+                   at_clamp(pl, pw, ph, x + 2, y));
+}
+
+static inline int half_v(const uint8_t* pl, int pw, int ph, int x, int y) {
+    return half_t4(at_clamp(pl, pw, ph, x, y - 1), at_clamp(pl, pw, ph, x, y), at_clamp(pl, pw, ph, x, y + 1),
+                   at_clamp(pl, pw, ph, x, y + 2));
+}
+
+static void build_half_planes(const uint8_t* prev, int pw, int ph, uint8_t* hH, uint8_t* hV, uint8_t* hD) {
+    for (int y = 0; y < ph; y++)
+        for (int x = 0; x < pw; x++) hH[(size_t)y * pw + x] = (uint8_t)half_h(prev, pw, ph, x, y);
+    for (int y = 0; y < ph; y++)
+        for (int x = 0; x < pw; x++) hV[(size_t)y * pw + x] = (uint8_t)half_v(prev, pw, ph, x, y);
+    for (int y = 0; y < ph; y++) {
+        const int ym = y == 0 ? 0 : y - 1, yp = y + 1 < ph ? y + 1 : ph - 1, y2 = y + 2 < ph ? y + 2 : ph - 1;
+        for (int x = 0; x < pw; x++)
+            hD[(size_t)y * pw + x] = (uint8_t)half_t4(hH[(size_t)ym * pw + x], hH[(size_t)y * pw + x],
+                                                      hH[(size_t)yp * pw + x], hH[(size_t)y2 * pw + x]);
+    }
+}
+
+/* prediction at (ox + qx/4, oy + qy/4), qx, qy in [-2, 2], from the half planes */
+static void mc_block_q(const uint8_t* prev, const uint8_t* hH, const uint8_t* hV, const uint8_t* hD, int pw, int ph,
+                       int ox, int oy, int qx, int qy, uint8_t out[64]) {
+    const int sx = qx < 0 ? -1 : 0, sy = qy < 0 ? -1 : 0;
+    for (int r = 0; r < 8; r++)
+        for (int c = 0; c < 8; c++) {
+            const int x = ox + c, y = oy + r;
+            int v;
+            if (!qx && !qy) v = at_clamp(prev, pw, ph, x, y);
+            else if (!qy && abs(qx) == 2) v = at_clamp(hH, pw, ph, x + (qx < 0 ? -1 : 0), y);
+            else if (!qx && abs(qy) == 2) v = at_clamp(hV, pw, ph, x, y + (qy < 0 ? -1 : 0));
+            else if (abs(qx) == 2 && abs(qy) == 2) v = at_clamp(hD, pw, ph, x + (qx < 0 ? -1 : 0), y + (qy < 0 ? -1 : 0));
+            else if (!qy) v = (at_clamp(prev, pw, ph, x, y) + at_clamp(hH, pw, ph, qx == -1 ? x - 1 : x, y) + 1) / 2;
+            else if (!qx) v = (at_clamp(prev, pw, ph, x, y) + at_clamp(hV, pw, ph, x, qy == -1 ? y - 1 : y) + 1) / 2;
+            else
+                v = (at_clamp(prev, pw, ph, x, y) + at_clamp(hH, pw, ph, x + sx, y) + at_clamp(hV, pw, ph, x, y + sy) +
+                     at_clamp(hD, pw, ph, x + sx, y + sy) + 2) / 4;
+            out[r * 8 + c] = (uint8_t)clampi(v, 0, 255);
+        }
+}
+
+/* the same prediction computed directly from the reference plane (the
+ * definition the half planes must reproduce) */
+int csc_interp_q(const uint8_t* prev, int pw, int ph, int ox, int oy, int qx, int qy, int direct, uint8_t* out) {
+    if (!prev || !out || pw < 1 || ph < 1 || qx < -2 || qx > 2 || qy < -2 || qy > 2) return CSC_E_ARGS;
+    if (!direct) {
+        uint8_t* hp = (uint8_t*)malloc((size_t)3 * pw * ph);
+        if (!hp) return CSC_E_ARGS;
+        build_half_planes(prev, pw, ph, hp, hp + (size_t)pw * ph, hp + (size_t)2 * pw * ph);
+        mc_block_q(prev, hp, hp + (size_t)pw * ph, hp + (size_t)2 * pw * ph, pw, ph, ox, oy, qx, qy, out);
+        free(hp);
+        return CSC_OK;
+    }
+    for (int r = 0; r < 8; r++)
+        for (int c = 0; c < 8; c++) {
+            const int x = ox + c, y = oy + r;
+            const int sx = qx < 0 ? -1 : 0, sy = qy < 0 ? -1 : 0;
+            int v;
+            if (!qx && !qy) v = at_clamp(prev, pw, ph, x, y);
+            else if (!qy && abs(qx) == 2) v = half_h(prev, pw, ph, x + sx, y);
+            else if (!qx && abs(qy) == 2) v = half_v(prev, pw, ph, x, y + sy);
+            else if (abs(qx) == 2 && abs(qy) == 2)
+                v = half_t4(half_h(prev, pw, ph, x + sx, y + sy - 1), half_h(prev, pw, ph, x + sx, y + sy),
+                            half_h(prev, pw, ph, x + sx, y + sy + 1), half_h(prev, pw, ph, x + sx, y + sy + 2));
+            else if (!qy) v = (at_clamp(prev, pw, ph, x, y) + half_h(prev, pw, ph, qx == -1 ? x - 1 : x, y) + 1) / 2;
+            else if (!qx) v = (at_clamp(prev, pw, ph, x, y) + half_v(prev, pw, ph, x, qy == -1 ? y - 1 : y) + 1) / 2;
+            else
+                v = (at_clamp(prev, pw, ph, x, y) + half_h(prev, pw, ph, x + sx, y) + half_v(prev, pw, ph, x, y + sy) +
+                     half_t4(half_h(prev, pw, ph, x + sx, y + sy - 1), half_h(prev, pw, ph, x + sx, y + sy),
+                             half_h(prev, pw, ph, x + sx, y + sy + 1), half_h(prev, pw, ph, x + sx, y + sy + 2)) + 2) / 4;
+            out[r * 8 + c] = (uint8_t)clampi(v, 0, 255);
+        }
+    return CSC_OK;
+}
+
+/* ---- intra prediction (g[16], g[19]): src/codec.zig:1009-1256 */
+
+enum {
+    INTRA_NONE = 0, INTRA_DC, INTRA_PLANAR, INTRA_VERTICAL, INTRA_HORIZONTAL, INTRA_DIAG_DR, INTRA_DIAG_DL,
+    INTRA_HOR_DOWN, INTRA_SMOOTH, INTRA_PAETH, INTRA_CFL_N2, INTRA_CFL_N1, INTRA_CFL_P1, INTRA_CFL_P2,
+};
+
+/* the enabled modes for a plane kind: NONE first, then the tier's angular
+ * set, then the four chroma-from-luma variants when that gene is on */
+static int intra_modes(const pheno* p, int chroma, uint8_t modes[14]) {
+    static const int tier_n[5] = {1, 2, 5, 8, 10};
+    const float v = p->intra_gene;
+    const int tier = v < 0.25f ? 0 : v < 0.55f ? 1 : v < 0.85f ? 2 : v < 0.925f ? 3 : 4;
+    int n = 0;
+    for (int m = 0; m < tier_n[tier]; m++) modes[n++] = (uint8_t)m;
+    if (chroma && p->cfl)
+        for (int m = INTRA_CFL_N2; m <= INTRA_CFL_P2; m++) modes[n++] = (uint8_t)m;
+    return n;
+}
+
+static inline int mode_bits(int n) {
+    if (n <= 1) return 0;
+    int b = 0;
+    while ((1 << b) < n) b++;
+    return b; /* ceil(log2 n) */
+}
+
+static uint8_t dc_pred8(const uint8_t* rc, int pw, int sx, int sy) {
+    int dc = 128 * 16;
+    int s = 0;
+    if (sy > 0 && sx > 0) {
+        for (int i = 0; i < 8; i++) s += rc[(size_t)(sy - 1) * pw + sx + i] + rc[(size_t)(sy + i) * pw + sx - 1];
+        dc = (s + 8) / 16;
+    } else if (sy > 0) {
+        for (int i = 0; i < 8; i++) s += rc[(size_t)(sy - 1) * pw + sx + i];
+        dc = (s + 4) / 8;
+    } else if (sx > 0) {
+        for (int i = 0; i < 8; i++) s += rc[(size_t)(sy + i) * pw + sx - 1];
+        dc = (s + 4) / 8;
+    }
+    return (uint8_t)clampi(dc, 0, 255);
+}
+
+static void intra_predict(int mode, const uint8_t* rc, int pw, int ph, int sx, int sy, uint8_t out[64],
+                          const uint8_t* luma, int lpw) {
+    uint8_t top[8], left[8], tl = 128, tr = 128, bl = 128;
+    memset(top, 128, 8);
+    memset(left, 128, 8);
+    if (sy > 0) {
+        for (int c = 0; c < 8; c++) top[c] = rc[(size_t)(sy - 1) * pw + sx + c];
+        tl = sx > 0 ? rc[(size_t)(sy - 1) * pw + sx - 1] : 128;
+        tr = sx + 8 < pw ? rc[(size_t)(sy - 1) * pw + sx + 8] : top[7];
+    }
+    if (sx > 0) {
+        for (int r = 0; r < 8; r++) left[r] = rc[(size_t)(sy + r) * pw + sx - 1];
+        bl = sy + 8 < ph ? rc[(size_t)(sy + 8) * pw + sx - 1] : left[7];
+    }
+    switch (mode) {
+    case INTRA_NONE: memset(out, 128, 64); break;
+    case INTRA_VERTICAL:
+        for (int r = 0; r < 8; r++) memcpy(out + r * 8, top, 8);
+        break;
+    case INTRA_HORIZONTAL:
+        for (int r = 0; r < 8; r++) memset(out + r * 8, left[r], 8);
+        break;
+    case INTRA_DC: memset(out, dc_pred8(rc, pw, sx, sy), 64); break;
+    case INTRA_PLANAR:
+        for (int r = 0; r < 8; r++)
+            for (int c = 0; c < 8; c++)
+                out[r * 8 + c] = (uint8_t)clampi(((7 - c) * left[r] + (c + 1) * tr + (7 - r) * top[c] + (r + 1) * bl + 8) >> 4,
+                                                 0, 255);
+        break;
+    case INTRA_DIAG_DR: {
+        uint8_t ref[16];
+        memset(ref, 128, 16);
+        ref[0] = tl;
+        memcpy(ref + 1, top, 8);
+        memcpy(ref + 9, left, 7);
+        for (int r = 0; r < 8; r++)
+            for (int c = 0; c < 8; c++) out[r * 8 + c] = ref[r + c + 1 < 15 ? r + c + 1 : 15];
+        break;
+    }
+    case INTRA_DIAG_DL:
+        for (int r = 0; r < 8; r++)
+            for (int c = 0; c < 8; c++) {
+                const int i = c + (r + 1) / 2;
+                out[r * 8 + c] = top[i < 7 ? i : 7];
+            }
+        break;
+    case INTRA_HOR_DOWN:
+        for (int r = 0; r < 8; r++)
+            for (int c = 0; c < 8; c++) {
+                const int i = r + (c + 1) / 2;
+                out[r * 8 + c] = left[i < 7 ? i : 7];
+            }
+        break;
+    case INTRA_SMOOTH:
+        for (int r = 0; r < 8; r++)
+            for (int c = 0; c < 8; c++) {
+                const int hz = ((7 - c) * left[r] + (c + 1) * tr + 4) / 8;
+                const int vt = ((7 - r) * top[c] + (r + 1) * bl + 4) / 8;
+                out[r * 8 + c] = (uint8_t)clampi((hz + vt + 1) >> 1, 0, 255);
+            }
+        break;
+    case INTRA_PAETH:
+        for (int r = 0; r < 8; r++)
+            for (int c = 0; c < 8; c++) {
+                const int pl = left[r] + top[c] - tl;
+                const int dl = abs(pl - left[r]), dt = abs(pl - top[c]), dtl = abs(pl - tl);
+                out[r * 8 + c] = (dl <= dt && dl <= dtl) ? left[r] : (dt <= dtl ? top[c] : tl);
+            }
+        break;
+    default: { /* chroma from luma: 2x2-mean luma minus its mean, times alpha, on the chroma DC */
+        int ld[64];
+        int64_t mean = 0;
+        for (int r = 0; r < 8; r++) {
+            const uint8_t* l0 = luma + (size_t)(2 * sy + 2 * r) * lpw + 2 * sx;
+            const uint8_t* l1 = l0 + lpw;
+            for (int c = 0; c < 8; c++) {
+                ld[r * 8 + c] = (l0[2 * c] + l0[2 * c + 1] + l1[2 * c] + l1[2 * c + 1]) >> 2;
+                mean += ld[r * 8 + c];
+            }
+        }
+        const int lm = (int)((mean + 32) / 64);
+        const int dc = dc_pred8(rc, pw, sx, sy);
+        const int a2 = mode == INTRA_CFL_N2 ? -2 : mode == INTRA_CFL_N1 ? -1 : mode == INTRA_CFL_P1 ? 1 : 2;
+        for (int k = 0; k < 64; k++) out[k] = (uint8_t)clampi(dc + a2 * (ld[k] - lm) / 2, 0, 255);
+        break;
+    }
+    }
+}
+
+int csc_intra_predict(int mode, const uint8_t* plane, int pw, int ph, int sx, int sy, const uint8_t* luma, int lpw,
+                      uint8_t* out) {
+    if (!plane || !out || mode < 0 || mode > INTRA_CFL_P2 || sx < 0 || sy < 0 || sx + 8 > pw || sy + 8 > ph ||
+        (mode >= INTRA_CFL_N2 && !luma))
+        return CSC_E_ARGS;
+    intra_predict(mode, plane, pw, ph, sx, sy, out, luma, lpw);
+    return CSC_OK;
+}
+
+/* ---- CDEF-lite (g[20]): src/codec.zig:1320-1400 */
+
+static void cdef_plane(uint8_t* p, int pw, int ph, float qp, uint8_t* snap) {
+    static const int dirs[4][2] = {{1, 0}, {1, 1}, {0, 1}, {-1, 1}};
+    memcpy(snap, p, (size_t)pw * ph);
+    const int damp = 2 + (int)qp / 24;
+    for (int by = 0; by < ph / 8; by++)
+        for (int bx = 0; bx < pw / 8; bx++) {
+            const int sx = bx * 8, sy = by * 8;
+            int best_dir = 0;
+            int64_t best_cost = INT64_MAX;
+            for (int di = 0; di < 4; di++) {
+                int64_t cost = 0;
+                for (int r = 2; r < 6; r++)
+                    for (int c = 2; c < 6; c++) {
+                        const int x = sx + c, y = sy + r, ctr = snap[(size_t)y * pw + x];
+                        for (int k = 1; k <= 2; k++) {
+                            const int smp = at_clamp(snap, pw, ph, x + dirs[di][0] * k, y + dirs[di][1] * k);
+                            cost += abs(smp - ctr) / 4;
+                        }
+                    }
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_dir = di;
+                }
+            }
+            if (best_cost > 512) continue;
+            const int dx = dirs[best_dir][0], dy = dirs[best_dir][1];
+            for (int r = 0; r < 8; r++)
+                for (int c = 0; c < 8; c++) {
+                    const int x = sx + c, y = sy + r, ctr = snap[(size_t)y * pw + x];
+                    int64_t sum = ctr;
+                    for (int k = 1; k <= 2; k++) {
+                        const int kk = k < damp ? k : damp;
+                        for (int sg = 1; sg >= -1; sg -= 2) {
+                            const int smp = at_clamp(snap, pw, ph, x + dx * kk * sg, y + dy * kk * sg);
+                            sum += ctr + clampi(smp - ctr, -6, 6);
+                        }
+                    }
+                    p[(size_t)y * pw + x] = (uint8_t)clampi((int)((sum + 2) / 5), 0, 255);
+                }
+        }
+}
+
 /* SAD of an 8x8 block; the fast form samples every 2nd row and column and
  * doubles the sum (src/codec.zig:2239-2251) */
 static inline uint64_t sad_block(const uint8_t* a, const uint8_t* b, size_t stride, int fast) {
@@ -649,6 +936,12 @@ static inline void deblock_pair(uint8_t* p0, uint8_t* q0, uint8_t p1, uint8_t q1
     const float delta = clampf(filt, -maxd, maxd);
     *p0 = (uint8_t)clampf(roundf(fp0 + delta), 0.0f, 255.0f);
     *q0 = (uint8_t)clampf(roundf(fq0 - delta), 0.0f, 255.0f);
+}
+
+int csc_deblock_pair(uint8_t* p0, uint8_t* q0, uint8_t p1, uint8_t q1, float alpha, float beta) {
+    if (!p0 || !q0) return CSC_E_ARGS;
+    deblock_pair(p0, q0, p1, q1, alpha, beta);
+    return CSC_OK;
 }
 
 static void deblock_plane(uint8_t* pl, int pw, int ph, float strength, float qp) {
@@ -685,7 +978,21 @@ typedef struct codec {
     uint64_t dct_calls, idct_calls;
     float qp_adj, cur_gop;
     size_t prev_frame_bytes, last_frame_bytes;
+    uint8_t *mcH, *mcV, *mcD; /* quarter-pel half planes of the current reference plane */
+    uint8_t* snap;            /* CDEF snapshot */
+    const uint8_t* dec_cur_y; /* decoder: the frame's luma recon (chroma-from-luma source) */
 } codec;
+
+/* in-loop filters of a finished frame, identical on both sides: deblocking,
+ * then CDEF-lite (src/codec.zig:1543-1549, 2281-2284) */
+static void post_filters(const pheno* p, uint8_t* pl[3], int w, int h, float qp, uint8_t* snap) {
+    deblock_frame(p, pl, w, h, qp);
+    if (p->cdef) {
+        cdef_plane(pl[0], w, h, qp, snap);
+        cdef_plane(pl[1], w / 2, h / 2, qp, snap);
+        cdef_plane(pl[2], w / 2, h / 2, qp, snap);
+    }
+}
 
 static inline float frame_gop(uint32_t fi, uint32_t key_int) { return (float)(fi % key_int) / (float)key_int; }
 
@@ -702,8 +1009,22 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
     cx->scratch.n = 0;
     blockw w = {p->entropy, &cx->scratch, {0}};
     if (w.ac) rc_enc_init(&w.e, &cx->scratch);
+    if (p->qpel && !is_i) build_half_planes(prev, pw, ph, cx->mcH, cx->mcV, cx->mcD);
+    /* intra modes: a bit-packed map of the chosen mode per intra block, framed
+     * before the block data; only whole bytes are kept, the map's length is
+     * fixed by the block count (src/codec.zig:1627-1636, 2227-2231) */
+    uint8_t modes[14];
+    const int nmodes = intra_modes(p, chroma, modes);
+    const int mwidth = mode_bits(nmodes);
+    const size_t map_len = ((size_t)nbx * nby * mwidth + 7) / 8;
+    uint8_t mmap[1024];
+    memset(mmap, 0, sizeof mmap);
+    uint32_t mbacc = 0;
+    int mbn = 0;
+    size_t mmap_n = 0;
+    const uint8_t* luma = chroma ? cx->enc[0] : NULL; /* the frame's luma recon before deblocking */
 
-    uint8_t blk[64], pred[64], res[64], rec[64], hp[64];
+    uint8_t blk[64], pred[64], res[64], rec[64], hp[64], trial[64];
     int32_t lv[64];
     float deq[64];
     for (int by = 0; by < nby; by++) {
@@ -719,14 +1040,53 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
                 if (!is_i) bw_marker(&w, 2);
                 band_steps(p, qp, iq_scale, &rc, qb.q);
                 qb.dz0 = p->deadzone * reg_mod(p->gain[0], rc.mv_n);
-                quantize_block(p, &qb, blk, lv);
+                if (nmodes == 1) { /* raw-DCT intra, no mode code */
+                    quantize_block(p, &qb, blk, lv);
+                    cx->dct_calls++;
+                    for (int k = 0; k < 64; k++)
+                        if (lv[k]) last = k;
+                    bw_levels(&w, lv, last);
+                    dequant(qb.q, lv, deq);
+                    idct8(deq, enc + (size_t)sy * pw + sx, (size_t)pw);
+                    cx->idct_calls++;
+                    continue;
+                }
+                /* mode by the true pre-quantization error of the u8 residual */
+                int mode_i = 0;
+                uint64_t best_err = UINT64_MAX;
+                for (int mi = 0; mi < nmodes; mi++) {
+                    intra_predict(modes[mi], enc, pw, ph, sx, sy, trial, luma, cx->w);
+                    uint64_t err = 0;
+                    for (int k = 0; k < 64; k++) {
+                        const int sv = blk[k], pv = trial[k];
+                        err += (uint64_t)abs(pv + clampi(sv - pv, -128, 127) - sv);
+                    }
+                    if (err < best_err) {
+                        best_err = err;
+                        mode_i = mi;
+                        memcpy(pred, trial, 64);
+                    }
+                }
+                mbacc = (mbacc << mwidth) | (uint32_t)mode_i;
+                mbn += mwidth;
+                while (mbn >= 8) {
+                    mbn -= 8;
+                    if (mmap_n < sizeof mmap) mmap[mmap_n] = (uint8_t)(mbacc >> mbn);
+                    mmap_n++;
+                }
+                for (int k = 0; k < 64; k++) res[k] = (uint8_t)clampi((int)blk[k] - (int)pred[k] + 128, 0, 255);
+                quantize_block(p, &qb, res, lv);
                 cx->dct_calls++;
                 for (int k = 0; k < 64; k++)
                     if (lv[k]) last = k;
                 bw_levels(&w, lv, last);
                 dequant(qb.q, lv, deq);
-                idct8(deq, enc + (size_t)sy * pw + sx, (size_t)pw);
+                idct8(deq, rec, 8);
                 cx->idct_calls++;
+                uint8_t* dst = enc + (size_t)sy * pw + sx;
+                for (int r = 0; r < 8; r++)
+                    for (int c = 0; c < 8; c++)
+                        dst[(size_t)r * pw + c] = (uint8_t)clampi((int)pred[r * 8 + c] + (int)rec[r * 8 + c] - 128, 0, 255);
                 continue;
             }
             /* integer motion search against the reference, raster order, first minimum */
@@ -749,9 +1109,45 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
             const float perpx = (float)best / 64.0f;
             rc.mv_n = fminf(1.0f, ((float)abs(bdx) + (float)abs(bdy)) / 8.0f);
             const float skip_th = p->skip_th * reg_mod(p->gain[4], rc.mv_n);
-            uint32_t subpel = 4;
+            const uint32_t center = p->qpel ? 12u : 4u;
+            uint32_t subpel = center;
             const int ox = sx + bdx, oy = sy + bdy;
-            if (p->halfpel) {
+            if (p->qpel) {
+                /* the 8 half positions, then the quarter ring around the winning half */
+                uint64_t hb = best;
+                int bqx = 0, bqy = 0;
+                for (int qy = -2; qy <= 2; qy += 2)
+                    for (int qx = -2; qx <= 2; qx += 2) {
+                        if (!qx && !qy) continue;
+                        mc_block_q(prev, cx->mcH, cx->mcV, cx->mcD, pw, ph, ox, oy, qx, qy, hp);
+                        const uint64_t s2 = sad_block(blk, hp, 8, cx->fast);
+                        if (s2 < hb) {
+                            hb = s2;
+                            bqx = qx;
+                            bqy = qy;
+                        }
+                    }
+                if (bqx || bqy) {
+                    /* the ring moves with the running best, as in src/codec.zig:2075-2095 */
+                    for (int qy = -1; qy <= 1; qy++)
+                        for (int qx = -1; qx <= 1; qx++) {
+                            if (!qx && !qy) continue;
+                            const int cxq = bqx + qx, cyq = bqy + qy;
+                            if (abs(cxq) > 2 || abs(cyq) > 2) continue;
+                            if (abs(cxq) != 1 && abs(cyq) != 1) continue;
+                            mc_block_q(prev, cx->mcH, cx->mcV, cx->mcD, pw, ph, ox, oy, cxq, cyq, hp);
+                            const uint64_t s2 = sad_block(blk, hp, 8, cx->fast);
+                            if (s2 < hb) {
+                                hb = s2;
+                                bqx = cxq;
+                                bqy = cyq;
+                            }
+                        }
+                    subpel = (uint32_t)((bqx + 2) * 5 + (bqy + 2));
+                    best = hb;
+                    mc_block_q(prev, cx->mcH, cx->mcV, cx->mcD, pw, ph, ox, oy, bqx, bqy, pred);
+                }
+            } else if (p->halfpel) {
                 uint64_t hb = best;
                 int bhx = 0, bhy = 0;
                 for (int hy = -1; hy <= 1; hy++)
@@ -775,9 +1171,9 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
                 bw_marker(&w, 1); /* skip: the block keeps the co-located reference pixels */
                 continue;
             }
-            bw_marker(&w, subpel != 4 ? 2 : 0);
+            bw_marker(&w, subpel != center ? 2 : 0);
             bw_mv(&w, bdx, bdy);
-            if (subpel != 4) bw_subpel(&w, subpel);
+            if (subpel != center) bw_subpel(&w, subpel, p->qpel ? 5 : 4);
             else
                 for (int r = 0; r < 8; r++) memcpy(pred + r * 8, prev + (size_t)(oy + r) * pw + ox, 8);
             for (int k = 0; k < 64; k++) res[k] = (uint8_t)clampi((int)blk[k] - (int)pred[k] + 128, 0, 255);
@@ -796,6 +1192,10 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
                 for (int c = 0; c < 8; c++)
                     dst[(size_t)r * pw + c] = (uint8_t)clampi((int)pred[r * 8 + c] + (int)rec[r * 8 + c] - 128, 0, 255);
         }
+    }
+    if (mwidth > 0) {
+        if (map_len > sizeof mmap) cx->main.overflow = 1;
+        for (size_t i = 0; i < map_len && i < sizeof mmap; i++) put_u8(&cx->main, mmap[i]);
     }
     if (w.ac) rc_flush(&w.e);
     for (size_t i = 0; i < cx->scratch.n && i < cx->scratch.cap; i++) put_u8(&cx->main, cx->scratch.buf[i]);
@@ -819,7 +1219,7 @@ static void encode_frame(codec* cx, const uint8_t* yuv, uint32_t fi, uint8_t* re
     encode_plane(cx, fi, is_i, qp, sy, cx->enc[0], cx->prev[0], cx->w, cx->h, 0);
     encode_plane(cx, fi, is_i, qp, su, cx->enc[1], cx->prev[1], cx->w / 2, cx->h / 2, 1);
     encode_plane(cx, fi, is_i, qp, sv, cx->enc[2], cx->prev[2], cx->w / 2, cx->h / 2, 1);
-    deblock_frame(p, cx->enc, cx->w, cx->h, qp);
+    post_filters(p, cx->enc, cx->w, cx->h, qp, cx->snap);
     for (int c = 0; c < 3; c++) {
         uint8_t* t = cx->prev[c];
         cx->prev[c] = cx->enc[c];
@@ -846,6 +1246,20 @@ static int decode_plane(codec* cx, bytes_in* in, uint32_t fi, int is_i, float qp
     const int nbx = pw / 8, nby = ph / 8;
     const uint32_t rrow = (fi / p->refresh) % (uint32_t)(nby > 1 ? nby : 1);
     const int do_refresh = !is_i && !chroma && (fi / p->refresh) > 0;
+    if (p->qpel && !is_i) build_half_planes(prev, pw, ph, cx->mcH, cx->mcV, cx->mcD);
+    uint8_t modes[14], mmap[1024];
+    const int nmodes = intra_modes(p, chroma, modes);
+    const int mwidth = mode_bits(nmodes);
+    size_t map_len = 0;
+    if (mwidth > 0) {
+        map_len = ((size_t)nbx * nby * mwidth + 7) / 8;
+        if (map_len > sizeof mmap) return CSC_E_CORRUPT;
+        for (size_t i = 0; i < map_len; i++) mmap[i] = get_u8(in);
+    }
+    size_t map_p = 0;
+    uint32_t map_acc = 0;
+    int map_bn = 0;
+    const uint8_t* luma = chroma ? cx->dec_cur_y : NULL;
     blockr r = {p->entropy, in, {0}};
     if (r.ac) rc_dec_init(&r.d, in);
     uint8_t pred[64], rec[64];
@@ -860,9 +1274,28 @@ static int decode_plane(codec* cx, bytes_in* in, uint32_t fi, int is_i, float qp
             if (force_intra) {
                 if (!is_i) (void)br_marker(&r);
                 band_steps(p, qp, iq_scale, &rc, q);
+                if (nmodes == 1) {
+                    if (br_levels(&r, q, coef)) return CSC_E_CORRUPT;
+                    idct8(coef, cur + (size_t)sy * pw + sx, (size_t)pw);
+                    cx->idct_calls++;
+                    continue;
+                }
+                while (map_bn < mwidth) { /* MSB-first bits of the mode map */
+                    if (map_p >= map_len) return CSC_E_CORRUPT;
+                    map_acc = (map_acc << 8) | mmap[map_p++];
+                    map_bn += 8;
+                }
+                map_bn -= mwidth;
+                const uint32_t midx = (map_acc >> map_bn) & ((1u << mwidth) - 1u);
+                if ((int)midx >= nmodes) return CSC_E_CORRUPT;
+                intra_predict(modes[midx], cur, pw, ph, sx, sy, pred, luma, cx->w);
                 if (br_levels(&r, q, coef)) return CSC_E_CORRUPT;
-                idct8(coef, cur + (size_t)sy * pw + sx, (size_t)pw);
+                idct8(coef, rec, 8);
                 cx->idct_calls++;
+                uint8_t* dst = cur + (size_t)sy * pw + sx;
+                for (int rr = 0; rr < 8; rr++)
+                    for (int c = 0; c < 8; c++)
+                        dst[(size_t)rr * pw + c] = (uint8_t)clampi((int)pred[rr * 8 + c] + (int)rec[rr * 8 + c] - 128, 0, 255);
                 continue;
             }
             const int flags = br_marker(&r);
@@ -872,9 +1305,16 @@ static int decode_plane(codec* cx, bytes_in* in, uint32_t fi, int is_i, float qp
             rc.mv_n = fminf(1.0f, ((float)abs(dx) + (float)abs(dy)) / 8.0f);
             const int ox = sx + dx, oy = sy + dy;
             if (flags == 2) {
-                const uint32_t code = br_subpel(&r);
-                if (code >= 9) return CSC_E_CORRUPT;
-                interp_block(prev, pw, ph, ox, oy, (int)(code / 3) - 1, (int)(code % 3) - 1, pred);
+                if (p->qpel) {
+                    const uint32_t code = br_subpel(&r, 5);
+                    if (code >= 25) return CSC_E_CORRUPT;
+                    mc_block_q(prev, cx->mcH, cx->mcV, cx->mcD, pw, ph, ox, oy, (int)(code / 5) - 2, (int)(code % 5) - 2,
+                               pred);
+                } else {
+                    const uint32_t code = br_subpel(&r, 4);
+                    if (code >= 9) return CSC_E_CORRUPT;
+                    interp_block(prev, pw, ph, ox, oy, (int)(code / 3) - 1, (int)(code % 3) - 1, pred);
+                }
             } else {
                 if (ox < 0 || oy < 0 || ox + 8 > pw || oy + 8 > ph) return CSC_E_CORRUPT;
                 for (int rr = 0; rr < 8; rr++) memcpy(pred + rr * 8, prev + (size_t)(oy + rr) * pw + ox, 8);
@@ -1011,7 +1451,7 @@ int csc_eval_clip(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames, c
     cx.ysz = (size_t)w * h;
     cx.csz = cx.ysz / 4;
     const size_t fsz = cx.ysz + 2 * cx.csz;
-    uint8_t* planes = (uint8_t*)malloc(4 * fsz);
+    uint8_t* planes = (uint8_t*)malloc(4 * fsz + 4 * cx.ysz);
     uint8_t* stream = (uint8_t*)malloc(CLIP_STREAM_CAP + PLANE_SCRATCH_CAP);
     uint8_t* encout = (uint8_t*)malloc((size_t)frames * fsz); /* encoder recon, frame-major */
     uint8_t* decout = (uint8_t*)malloc((size_t)frames * fsz);
@@ -1021,6 +1461,10 @@ int csc_eval_clip(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames, c
         goto done;
     }
     memset(planes, 128, 4 * fsz);
+    cx.mcH = planes + 4 * fsz;
+    cx.mcV = cx.mcH + cx.ysz;
+    cx.mcD = cx.mcV + cx.ysz;
+    cx.snap = cx.mcD + cx.ysz;
     cx.enc[0] = planes;
     cx.enc[1] = planes + cx.ysz;
     cx.enc[2] = planes + cx.ysz + cx.csz;
@@ -1052,10 +1496,11 @@ int csc_eval_clip(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames, c
         const float qp = (float)get_u8(&in);
         const int is_i = ftype == 1;
         cx.cur_gop = frame_gop(fi, p.key_int);
+        cx.dec_cur_y = cur[0];
         rc = decode_plane(&cx, &in, fi, is_i, qp, cur[0], prv[0], cx.w, cx.h, 0);
         if (rc == CSC_OK) rc = decode_plane(&cx, &in, fi, is_i, qp, cur[1], prv[1], cx.w / 2, cx.h / 2, 1);
         if (rc == CSC_OK) rc = decode_plane(&cx, &in, fi, is_i, qp, cur[2], prv[2], cx.w / 2, cx.h / 2, 1);
-        deblock_frame(&p, cur, cx.w, cx.h, qp);
+        post_filters(&p, cur, cx.w, cx.h, qp, cx.snap);
         for (int c = 0; c < 3; c++) {
             memcpy(decout + fi * fsz + (c ? cx.ysz + (c - 1) * cx.csz : 0), cur[c], c ? cx.csz : cx.ysz);
             uint8_t* t = prv[c];

@@ -6,12 +6,12 @@
  * screen survivors.  The reference exposes this path only as the `arena eval`
  * subcommand (src/main.zig:219-351 evalAll -> src/codec.zig:2537 evalClip).
  *
- * Coverage: every tool that is off when its gene says "legacy" is restated,
- * plus the in-loop deblocking filter (g[17], src/codec.zig:1258-1310) and the
- * adaptive binary range coder (g[18] > 0.5, src/codec.zig:239-525).  Genomes
- * that switch on intra prediction (g[16] >= 0.25), chroma-from-luma
- * (g[19] > 0.5), CDEF (g[20] > 0.5), quarter-pel MC (g[21] > 0.5) or the 16x16
- * superblock layer (g[22] > 0.5) are rejected with CSC_E_UNSUPPORTED.
+ * Coverage: the 8x8 codec with every gene-gated tool -- intra prediction
+ * (g[16], src/codec.zig:1009-1256), deblocking (g[17], :1258-1310), the
+ * adaptive binary range coder (g[18], :239-525), chroma-from-luma (g[19]),
+ * CDEF-lite (g[20], :1320-1400) and quarter-pel MC (g[21], :813-1006).
+ * Genomes that switch on the 16x16 superblock layer (g[22] > 0.5) are
+ * rejected with CSC_E_UNSUPPORTED.
  *
  * All functions are thread-safe (no shared mutable state); buffers are
  * caller-owned.
@@ -70,6 +70,21 @@ float csc_psnr_plane(const uint8_t* a, const uint8_t* b, uint64_t n);
 /* putSvarint then getSvarint (src/codec.zig:165-168, 215-220): v for v >= 0,
  * v + 1 for v < 0 (the decode quirk described at csc_eval_clip). */
 int64_t csc_svarint_roundtrip(int64_t v);
+
+/* intraPredict (src/codec.zig:1118-1246): the 8x8 prediction of `mode`
+ * (0 NONE ... 13 CFL +2/2) at (sx, sy) of an 8-bit plane; `luma` (row pitch
+ * lpw) is the co-located luma plane, needed by the chroma-from-luma modes. */
+int csc_intra_predict(int mode, const uint8_t* plane, int pw, int ph, int sx, int sy, const uint8_t* luma, int lpw,
+                      uint8_t* out);
+
+/* Quarter-pel prediction at (ox + qx/4, oy + qy/4), qx, qy in [-2, 2]: from
+ * prebuilt half planes (direct == 0, mcBlockQ, src/codec.zig:959-1006, the
+ * path both codec sides use) or by the direct filters (direct != 0,
+ * interpBlockQ, :866-921). */
+int csc_interp_q(const uint8_t* prev, int pw, int ph, int ox, int oy, int qx, int qy, int direct, uint8_t* out);
+
+/* deblockPair (src/codec.zig:1258-1272) on one edge pair. */
+int csc_deblock_pair(uint8_t* p0, uint8_t* q0, uint8_t p1, uint8_t q1, float alpha, float beta);
 
 /* Adaptive range coder round trip over a symbol stream (src/codec.zig:
  * 279-358): codes `n` bits with contexts ctx[i] (< 64) into buf (capacity

@@ -1,13 +1,15 @@
 """The codec label source (CPU): corpus, encoder/decoder, metrics.
 
-Pinned by the reference's own data: the corpus against corpus/manifest.json,
-and every hall-of-fame record whose tools are restated against its published
-PSNR, bpp, dec_mops, chroma PSNR, MS-SSIM and MS-SSIM dB (halloffame/*.json,
-measured by the reference on the five legacy clips of tools/gen_corpus.py).
-Beside them: the closed-loop invariant of src/codec.zig:2833-2895, the range
-coder round trip of src/codec.zig:3001-3094, the MS-SSIM and PSNR properties of
-src/msssim.zig:186-226 and src/yuv.zig:82-85, and an FP64 MS-SSIM written from
-Wang et al. 2003.
+Pinned by the reference's own data: the corpus against corpus/manifest.json;
+all 22 hall-of-fame records against their published PSNR, bpp, dec_mops,
+chroma PSNR, MS-SSIM and MS-SSIM dB (halloffame/*.json, measured by the
+reference on the five legacy clips of tools/gen_corpus.py); the one
+leaderboard genome against its row on the ten-clip corpus.  Beside them: the
+closed-loop invariant of src/codec.zig:2833-2970, the unit-test known answers
+of src/codec.zig:2973-3124 (sub-pel geometry, intra prediction, deblocking),
+the range coder round trip of :3001-3094, the MS-SSIM and PSNR properties of
+src/msssim.zig:186-226 and src/yuv.zig:82-85, and an FP64 MS-SSIM written
+from Wang et al. 2003.
 """
 import hashlib
 import json
@@ -57,17 +59,16 @@ def test_halloffame_records_reproduce(cc, clips):
     recs = kats["records"] + [kats["champion"]]
     ev = cc.Evaluator(names=LEGACY5, corpus_dir=clips, msssim=True)
     gs = [_kat_genome(cc, r) for r in recs]
-    keep = [i for i, g in enumerate(gs) if cc.supported(g)]
-    assert len(keep) == len(recs) - 1  # one record switches CDEF on
-    ms = ev.evaluate([gs[i] for i in keep])
+    assert all(cc.supported(g) for g in gs)
+    ms = ev.evaluate(gs)
     checked = 0
-    for i, m in zip(keep, ms):
-        ref = recs[i]["metrics"]
+    for r, m in zip(recs, ms):
+        ref = r["metrics"]
         for k, d in PRINTED.items():
             if k in ref:
-                assert round(m[k], d) == pytest.approx(ref[k], abs=0.51 * 10 ** -d), (recs[i]["id"], k, m[k], ref[k])
+                assert round(m[k], d) == pytest.approx(ref[k], abs=0.51 * 10 ** -d), (r["id"], k, m[k], ref[k])
                 checked += 1
-    assert checked >= 100
+    assert checked >= 110
 
 
 def test_leaderboard_bot_reproduces_on_the_full_corpus(cc, clips):
@@ -85,34 +86,100 @@ def test_leaderboard_bot_reproduces_on_the_full_corpus(cc, clips):
 
 def test_unsupported_tools_rejected(cc):
     base = [0.5] * 16 + [0.0] * 7 + [0.5] * 6
-    for k, v in ((16, 0.25), (19, 0.51), (20, 0.51), (21, 0.51), (22, 0.51)):
+    g = list(base)
+    g[22] = 0.51
+    assert not cc.supported(cc.genome(g))
+    for k in range(16, 23):
         g = list(base)
-        g[k] = v
-        assert not cc.supported(cc.genome(g))
-    for k, v in ((16, 0.2499), (17, 1.0), (18, 1.0), (19, 0.5), (20, 0.5), (21, 0.5), (22, 0.5)):
-        g = list(base)
-        g[k] = v
+        g[k] = 0.5 if k == 22 else 1.0
         assert cc.supported(cc.genome(g))
     g = list(base)
-    g[20] = 0.9
+    g[22] = 0.9
     with pytest.raises(cc.CodecError):
         cc.eval_clip(bytes(320 * 192 * 3 // 2 * 2), cc.genome(g), frames=2)
 
 
 def test_closed_loop_holds_on_the_range_coder_stream(cc, clips):
     """Decoder output == encoder recon, byte for byte, with the adaptive range
-    coder on, half-pel, long GOPs, intra refresh, adaptive q and deblocking."""
+    coder on and every other tool in combination: the intra mode tiers,
+    chroma-from-luma, CDEF, quarter-pel or half-pel MC, deblocking, long
+    GOPs, intra refresh and adaptive q (src/codec.zig:2833-2970)."""
     from csfunnel import corpus
     data = corpus.load(clips)
     rng = np.random.default_rng(11)
-    for i in range(6):
-        g = list(np.round(rng.random(16), 4)) + [0.0, float(rng.random()), 0.8, 0.0, 0.0, 0.0, 0.0] + \
-            list(np.round(rng.random(6), 4))
+    for i in range(10):
+        g = list(np.round(rng.random(16), 4)) + [0.0] * 7 + list(np.round(rng.random(6), 4))
         g[12] = 0.9 if i % 2 == 0 else 0.1
+        g[16] = (0.1, 0.4, 0.7, 0.9, 0.95)[i % 5]
+        g[17], g[18] = float(rng.random()), 0.8
+        g[19] = 0.9 if i % 2 else 0.2
+        g[20] = 0.9 if i % 3 == 0 else 0.2
+        g[21] = 0.9 if i % 4 < 2 else 0.2
         gn = cc.genome(g, i)
-        for n in ("quake", "zoom"):
+        for n in ("quake", "edge"):
             st = cc.eval_clip(data[n], gn, flags=cc.FAST_SAD | cc.CLOSED_LOOP)
             assert st.closed_loop == 1 and st.idct_calls == 2 * st.dct_calls
+
+
+def test_interp_q_geometry(cc):
+    """src/codec.zig:2973-2999: every sub-pel position of a flat plane is the
+    flat value; the +2 (half) position of a linear ramp is the ramp midpoint;
+    the half-plane path equals the direct filters away from the border."""
+    import ctypes as C
+    L = cc.lib()
+    out = C.create_string_buffer(64)
+    flat = bytes([100]) * 1600
+    for qx in range(-2, 3):
+        for qy in range(-2, 3):
+            for direct in (0, 1):
+                assert L.csc_interp_q(flat, 40, 40, 8, 8, qx, qy, direct, out) == 0
+                assert set(out.raw) == {100}
+    ramp = bytes([50 + c * 2 for r in range(40) for c in range(40)])
+    for direct in (0, 1):
+        assert L.csc_interp_q(ramp, 40, 40, 8, 8, 2, 0, direct, out) == 0
+        assert list(out.raw) == [50 + (8 + c) * 2 + 1 for r in range(8) for c in range(8)]
+    noise = np.random.default_rng(2).integers(0, 256, 48 * 48).astype(np.uint8).tobytes()
+    o2 = C.create_string_buffer(64)
+    for qx in range(-2, 3):
+        for qy in range(-2, 3):
+            L.csc_interp_q(noise, 48, 48, 16, 20, qx, qy, 0, out)
+            L.csc_interp_q(noise, 48, 48, 16, 20, qx, qy, 1, o2)
+            assert out.raw == o2.raw, (qx, qy)
+
+
+def test_intra_predict_known_answers(cc):
+    """src/codec.zig:3095-3109 (vertical copies the top row, DC of a flat
+    plane is flat), plus horizontal, the NONE mode's mid-gray, the 128 fill
+    at the frame corner and chroma-from-luma on flat luma."""
+    import ctypes as C
+    L = cc.lib()
+    out = C.create_string_buffer(64)
+    rec = bytes([(r * 7 + c * 3) % 256 for r in range(32) for c in range(32)])
+    assert L.csc_intra_predict(3, rec, 32, 32, 8, 8, None, 0, out) == 0
+    assert list(out.raw) == [rec[7 * 32 + 8 + c] for r in range(8) for c in range(8)]
+    assert L.csc_intra_predict(4, rec, 32, 32, 8, 8, None, 0, out) == 0
+    assert list(out.raw) == [rec[(8 + r) * 32 + 7] for r in range(8) for c in range(8)]
+    assert L.csc_intra_predict(1, bytes([90]) * 1024, 32, 32, 8, 8, None, 0, out) == 0
+    assert set(out.raw) == {90}
+    assert L.csc_intra_predict(0, rec, 32, 32, 8, 8, None, 0, out) == 0 and set(out.raw) == {128}
+    assert L.csc_intra_predict(3, rec, 32, 32, 0, 0, None, 0, out) == 0 and set(out.raw) == {128}
+    luma = bytes([200]) * (64 * 64)
+    for mode in range(10, 14):
+        assert L.csc_intra_predict(mode, bytes([77]) * 1024, 32, 32, 8, 8, luma, 64, out) == 0
+        assert set(out.raw) == {77}
+    assert L.csc_intra_predict(10, rec, 32, 32, 8, 8, None, 0, out) < 0
+
+
+def test_deblock_pair_known_answers(cc):
+    """src/codec.zig:3111-3124: the midpoint is never crossed, flat stays flat."""
+    import ctypes as C
+    L = cc.lib()
+    p0, q0 = C.c_uint8(100), C.c_uint8(110)
+    L.csc_deblock_pair(C.byref(p0), C.byref(q0), 100, 110, 50.0, 25.0)
+    assert 100 < p0.value <= 105 and 105 <= q0.value < 110
+    a, b = C.c_uint8(77), C.c_uint8(77)
+    L.csc_deblock_pair(C.byref(a), C.byref(b), 77, 77, 50.0, 25.0)
+    assert a.value == b.value == 77
 
 
 def test_varint_stream_reproduces_the_reference_decode(cc, clips):
@@ -229,7 +296,7 @@ def test_codec_library_exports_every_declared_symbol(cc):
     txt = open(os.path.join(os.path.dirname(GOLDEN), "..", "include", "cscodec.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     names = sorted(set(re.findall(r"\b(csc_[a-z0-9_]+)\s*\(", txt)))
-    assert len(names) == 6
+    assert len(names) == 9
     L = cc.lib()
     for n in names:
         assert hasattr(L, n) and n in cc.SIGNATURES, n
