@@ -79,6 +79,19 @@ class Cfg:
                          self.sigma, self.jump_p, self.imm_p, 0.0)
 
 
+def caller_cfg(screens: int, pop: int, seed: int, shard: int, gen: int, threads: int = 4, keep: int = 0,
+               sigma_decay: float = 0.93) -> Cfg:
+    """The screen parameters brain/evolve.py passes for generation `gen`
+    (brain/evolve.py:368-371, 431-438; sigma_decay from brain/config.json:14):
+    K = keep or pop - max(2, pop // 8) - max(2, pop // 8); seed = seed 1000003 +
+    shard 131 + gen; sigma = "%.4f" % (0.12 sigma_decay^gen); jump "0.02";
+    imm "0.01"; every value read as the Zig CLI reads its flags."""
+    K = keep if keep > 0 else pop - max(2, pop // 8) - max(2, pop // 8)
+    return Cfg(parse_u64(str(screens)), parse_u64(str(K)), parse_u64(str(seed * 1000003 + shard * 131 + gen)),
+               parse_u64(str(threads)), parse_f32(f"{0.12 * (sigma_decay ** gen):.4f}"), parse_f32("0.02"),
+               parse_f32("0.01"))
+
+
 def parse_u64(tok: str) -> int:
     b = tok.encode()
     v = C.c_uint64()

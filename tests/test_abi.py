@@ -151,3 +151,13 @@ def test_screen_cmd_host_errors(tmp_path, golden, capsys):
     # bad integer
     assert scr.screen_cmd(["--lambda", "12x", "--model", m]) == 1
     assert not os.path.exists(out) and not os.path.exists(cnt)
+
+
+def test_caller_cfg_mirrors_evolve_call():
+    """brain/evolve.py:368-371, 431-438 for pop 256, seed 42, shard 7."""
+    cfg = scr.caller_cfg(2650000000, 256, 42, 7, 3)
+    assert (cfg.lambda_, cfg.keep, cfg.seed, cfg.threads) == (2650000000, 256 - 32 - 32, 42 * 1000003 + 7 * 131 + 3, 4)
+    assert cfg.sigma == float(np.float32(0.0965)) and "%.4f" % (0.12 * 0.93 ** 3) == "0.0965"
+    assert cfg.jump_p == float(np.float32(0.02)) and cfg.imm_p == float(np.float32(0.01))
+    assert scr.caller_cfg(10, 12, 1, 0, 0).keep == 12 - 2 - 2
+    assert scr.caller_cfg(10, 12, 1, 0, 0, keep=5).keep == 5
