@@ -10,9 +10,11 @@
 //
 // Kernels:
 //   k_gram        X'X, X'y and y'y in one pass: features are built per row in
-//                 LDS, [phi | y] (padded to 176 columns = 11 tiles of 16) is
-//                 multiplied with v_mfma_f64_16x16x4f64 into the 66 upper
-//                 tiles; each workgroup writes its partial tiles.
+//                 LDS (one multiply per column from the extended row
+//                 [g, y, 1, 0]; the next rows are fetched into registers
+//                 meanwhile), [phi | y] (padded to 176 columns = 11 tiles of
+//                 16) is multiplied with v_mfma_f64_16x16x4f64 into the 66
+//                 upper tiles; each workgroup writes its partial tiles.
 //   k_gram_reduce fixed-order sum of the partial tiles (deterministic).
 //   k_chol        Cholesky of X'X + a I in LDS (packed lower triangle) and the
 //                 two triangular solves, one workgroup.
@@ -28,7 +30,7 @@ constexpr int RT = RP / 16;
 constexpr int NTILE = RT * (RT + 1) / 2;  // 66 upper-triangle tiles
 constexpr int TPW = (NTILE + 3) / 4;      // tiles per wave (17)
 constexpr int GSB = 32;                   // rows per LDS sub-batch
-constexpr int GCH = 1024;                 // rows per workgroup
+constexpr int GCH = 512;                  // rows per workgroup
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
@@ -103,30 +105,67 @@ __device__ __forceinline__ void gram_store(const dbl4 (&acc)[TPW], double *part,
     }
 }
 
+// Column e of [phi | y] as the product of two entries a | b << 8 of the
+// extended row [g_0 .. g_{ng-1}, y, 1, 0]: every column is one multiply and
+// the values are those of feature_value (x * 1 and 0 * 0 are exact).
+__device__ __forceinline__ uint32_t feature_pair(int e, int ng, int p) {
+    const uint32_t Y = (uint32_t)ng, ONE = Y + 1u, ZERO = Y + 2u;
+    const uint32_t c = feature_code(e, ng, p), kind = c & 255u, a = (c >> 8) & 255u, b = c >> 16;
+    switch (kind) {
+    case 1: return ONE | (ONE << 8);
+    case 2: return a | (ONE << 8);
+    case 3: return a | (b << 8);
+    case 4: return Y | (ONE << 8);
+    default: return ZERO | (ZERO << 8);
+    }
+}
+
+constexpr int GXW = 32;                          // extended row width (ng + 3 <= 32)
+constexpr int GPF = (GSB * GXW + 255) / 256;     // prefetched values per thread
+
 __global__ __launch_bounds__(256) void k_gram(const double *__restrict__ genes, int ng, const double *__restrict__ y,
                                               uint64_t n, int p, double *__restrict__ part) {
     __shared__ double phi[GSB][RP];
-    __shared__ uint32_t code[RP];
-    __shared__ double grow[GSB][32];
-    __shared__ double yrow[GSB];
+    __shared__ uint32_t pair[RP];
+    __shared__ double xrow[GSB][GXW];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    for (int e = tid; e < RP; e += 256) code[e] = feature_code(e, ng, p);
+    for (int e = tid; e < RP; e += 256) pair[e] = feature_pair(e, ng, p);
     dbl4 acc[TPW];
 #pragma unroll
     for (int q = 0; q < TPW; ++q) acc[q] = dbl4{0.0, 0.0, 0.0, 0.0};
     const uint64_t r_begin = (uint64_t)blockIdx.x * GCH;
     const uint64_t r_end = min<uint64_t>(n, r_begin + GCH);
+    // the next sub-batch's genes, y and the constant 1 travel in registers
+    // while the current one is multiplied; rows past the end are all zero
+    double pf[GPF];
+    auto fetch = [&](uint64_t r0) {
+#pragma unroll
+        for (int k = 0; k < GPF; ++k) {
+            const int idx = tid + 256 * k, row = idx / GXW, col = idx % GXW;
+            const uint64_t r = r0 + (uint64_t)row;
+            double v = 0.0;
+            if (row < GSB && r < r_end) {
+                if (col < ng) v = genes[r * (uint64_t)ng + (uint64_t)col];
+                else if (col == ng) v = y[r];
+                else if (col == ng + 1) v = 1.0;
+            }
+            pf[k] = v;
+        }
+    };
+    fetch(r_begin);
     for (uint64_t r0 = r_begin; r0 < r_end; r0 += GSB) {
         __syncthreads();
-        for (int idx = tid; idx < GSB * ng; idx += 256) {
-            const int row = idx / ng, k = idx % ng;
-            grow[row][k] = (r0 + row < r_end) ? genes[(r0 + row) * ng + k] : 0.0;
+#pragma unroll
+        for (int k = 0; k < GPF; ++k) {
+            const int idx = tid + 256 * k;
+            if (idx < GSB * GXW) xrow[idx / GXW][idx % GXW] = pf[k];
         }
-        for (int row = tid; row < GSB; row += 256) yrow[row] = (r0 + row < r_end) ? y[r0 + row] : 0.0;
         __syncthreads();
+        if (r0 + GSB < r_end) fetch(r0 + GSB);
         for (int idx = tid; idx < GSB * RP; idx += 256) {
             const int row = idx / RP, e = idx % RP;
-            phi[row][e] = (r0 + row < r_end) ? feature_value(code[e], grow[row], yrow[row]) : 0.0;
+            const uint32_t pr = pair[e];
+            phi[row][e] = xrow[row][pr & 255u] * xrow[row][pr >> 8];
         }
         __syncthreads();
         switch (wave) {
