@@ -554,6 +554,41 @@ __device__ __forceinline__ void synth_genes(float *g, uint64_t s0, const float *
     }
 }
 
+// Records of the lanes of an MLP batch that pass the bound: the genome is
+// synthesized again in full (only genes 0..15 were needed for the score).
+// Out of line: batches emit rarely once the bounds are set, and keeping this
+// code out of the scoring loop keeps the loop's instruction footprint small.
+template <bool FAST_MUT>
+__device__ __attribute__((noinline)) void emit_mlp_records(Record *rec, uint32_t *rec_count, uint32_t rec_cap,
+                                                           const float *el, float sigma, float sig24, bool imm,
+                                                           uint64_t s0, uint32_t ia, uint32_t ib, uint32_t gi,
+                                                           float jval, float score, bool pass, uint64_t pm, int lane,
+                                                           uint32_t stream, uint64_t cidx, uint64_t c) {
+    float g[CS_NTOT];
+    if (imm) synth_genes<true, FAST_MUT, 0, CS_NTOT>(g, s0, el, ia, ib, sigma, sig24);
+    else synth_genes<false, FAST_MUT, 0, CS_NTOT>(g, s0, el, ia, ib, sigma, sig24);
+#pragma unroll
+    for (int k = 0; k < CS_NTOT; ++k) g[k] = (k == (int)gi) ? jval : g[k];
+    uint32_t basei = 0;
+    if (lane == 0) basei = atomicAdd(rec_count, (uint32_t)__popcll(pm));
+    basei = __shfl(basei, 0);
+    if (pass) {
+        const uint32_t slot = basei + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull));
+        if (slot < rec_cap) {
+            Record *o = rec + slot;
+            o->score = score;
+            o->stream = stream;
+            o->index = cidx;
+            o->start = c;
+            o->ia = ia;
+            o->ib = ib;
+#pragma unroll
+            for (int k = 0; k < CS_NTOT; ++k) o->g[k] = g[k];
+            o->_pad = 0;
+        }
+    }
+}
+
 // IMM: 0 normal candidates, 1 immigrants, 2 either, by the uniform imm_rt
 // (one inlined copy of the scorer for both kinds: the MLP variants).
 template <int IMM, bool LDS_ELITES, bool FAST_MUT, int SC, bool DUMP>
@@ -631,6 +666,12 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
     }
     const bool pass = act && !(score <= bound);
     const uint64_t pm = __ballot(pass);
+    if constexpr (KH < CS_NTOT && !DUMP) {
+        if (pm)
+            emit_mlp_records<FAST_MUT>(a.rec, a.rec_count, a.rec_cap, el, a.sigma, a.sig24, imm, s0, ia, ib, gi, jval,
+                                       score, pass, pm, lane, stream, cidx, c);
+        return;
+    }
     if constexpr (KH < CS_NTOT) {
         if (DUMP || pm) {
             if (imm) synth_genes<true, FAST_MUT, 0, CS_NTOT>(g, s0, el, ia, ib, a.sigma, a.sig24);
