@@ -436,6 +436,17 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
     const int q = lane >> 4, j = lane & 15;
     typedef typename std::conditional<SC == SC_MLP_FP8, uint64_t, bf16x8>::type Frag;
     const h2 q1 = h2s((_Float16)w.q1), q2 = h2s((_Float16)GELU_Q);
+    // BF16: W1 fragments are fetched afresh for every batch (an opaque copy
+    // of the pointer): held across batches they would be 32 more live
+    // registers in the scoring loop and get spilled to scratch, whose reloads
+    // then stall every layer-1 MFMA.  The FP8 variant (fewer registers) keeps
+    // them.
+    uint64_t w1p = (uint64_t)ml.w1f;
+    if constexpr (SC == SC_MLP_BF16) asm volatile("" : "+s"(w1p));
+    const bf16x4 *w1f = (const bf16x4 *)w1p;
+    bf16x4 w1v[16];
+#pragma unroll
+    for (int mt = 0; mt < 16; ++mt) w1v[mt] = w1f[mt * 64 + lane];
     Frag af[4][8];
 #pragma unroll
     for (int c4 = 0; c4 < 4; ++c4) {
@@ -443,7 +454,7 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
 #pragma unroll
         for (int mt = 0; mt < 16; ++mt) {
             f32x4 acc = *reinterpret_cast<const f32x4 *>(ml.b1 + 16 * mt + 4 * q);
-            const bf16x4 wa = ml.w1f[mt * 64 + lane];
+            const bf16x4 wa = w1v[mt];
             acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const s16x4 *>(&wa), bx, acc, 0, 0, 0);
             const h2 x01 = pk_rtz(acc[0], acc[1]), x23 = pk_rtz(acc[2], acc[3]);
             const h2 g01 = x01 * gelu_sig_h2(x01, q1), g23 = x23 * gelu_sig_h2(x23, q1);
