@@ -141,9 +141,10 @@ def test_training_beats_quadratic_fallback(dev, trained):
 
 
 def test_funnel_survivors_oracle_verified(dev, trained):
-    """Config-4 funnel: the quadratic and the BF16 MLP screens run 2.65e9
-    candidates per generation from the same elites; the codec label source
-    re-scores each screen's top 64 survivors."""
+    """Config-4 funnel: the quadratic, the BF16 MLP and the MLP+quadratic
+    ensemble screens run 2.65e9 candidates per generation from the same
+    elites; the codec label source re-scores each screen's top 64 survivors,
+    which must beat the elites they were bred from."""
     from csfunnel import codec, screen as scr, surrogate
     m, w = trained[0], trained[1]
     el = scr.load_elites(os.path.join(REPO, "tests", "golden", "c1_elites.txt"))
@@ -151,9 +152,10 @@ def test_funnel_survivors_oracle_verified(dev, trained):
     res = surrogate.funnel_compare(dev, el, m, w, codec.Evaluator(), gens=2)
     for g in res["generations"]:
         print({k: round(v, 3) if isinstance(v, float) else v for k, v in g.items()}, flush=True)
-    g0 = res["generations"][0]
-    assert g0["quad_mean"] > res["elites_mean"] and g0["mlp_mean"] > res["elites_mean"]
-    assert sum(g["mlp_mean"] for g in res["generations"]) > sum(g["quad_mean"] for g in res["generations"])
+    print(f"elites mean {res['elites_mean']:.3f} max {res['elites_max']:.3f}", flush=True)
+    for g in res["generations"]:
+        for arm in ("quad", "mlp", "ensemble"):
+            assert g[f"{arm}_mean"] > res["elites_mean"], (g["gen"], arm)
 
 
 def _screen_recall(dev, m, scorer, lam, keep):
