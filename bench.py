@@ -264,7 +264,10 @@ def main():
             from csfunnel import codec, surrogate
             print("funnel: quadratic vs MLP screen, survivors re-scored by the codec label source", file=sys.stderr,
                   flush=True)
-            funnel = surrogate.funnel_compare(dev, elites, mlp, wq, codec.Evaluator(), gens=args.funnel_gens)
+            funnel = surrogate.funnel_compare(
+                dev, elites, mlp, wq, codec.Evaluator(), gens=args.funnel_gens,
+                progress=lambda r: print("funnel gen %d: quad %.3f mlp %.3f ensemble %.3f" % (
+                    r["gen"], r["quad_mean"], r["mlp_mean"], r["ensemble_mean"]), file=sys.stderr, flush=True))
     dev.close()
     ms_score = sum(s["ms_score"] for s in stats)
     ms_map = sum(s["ms_map"] for s in stats)

@@ -34,7 +34,7 @@ class ClipStats(C.Structure):
     _fields_ = [("bytes", C.c_uint64), ("psnr_y", C.c_float), ("psnr_u", C.c_float), ("psnr_v", C.c_float),
                 ("msssim_y", C.c_float), ("enc_mops", C.c_float), ("dec_mops", C.c_float),
                 ("dct_calls", C.c_uint64), ("idct_calls", C.c_uint64), ("closed_loop", C.c_int32),
-                ("_pad", C.c_int32)]
+                ("enc_ms", C.c_float)]
 
 
 class CodecError(RuntimeError):
@@ -47,6 +47,8 @@ SIGNATURES = {
     "csc_genome_supported": (C.c_int, [C.POINTER(Genome)]),
     "csc_eval_clip": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Genome), C.c_uint32,
                                 C.POINTER(ClipStats)]),
+    "csc_eval_clip_ex": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Genome), C.c_uint32,
+                                   C.c_float, C.c_void_p, C.POINTER(ClipStats)]),
     "csc_plane_msssim": (C.c_int, [C.c_char_p, C.c_char_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
     "csc_psnr_plane": (C.c_float, [C.c_char_p, C.c_char_p, C.c_uint64]),
     "csc_svarint_roundtrip": (C.c_int64, [C.c_int64]),
@@ -105,9 +107,19 @@ def supported(g: Genome) -> bool:
 
 
 def eval_clip(data: bytes, g: Genome, w: int = _corpus.W, h: int = _corpus.H, frames: int = _corpus.FRAMES,
-              flags: int = FAST_SAD) -> ClipStats:
+              flags: int = FAST_SAD, qscale: float = 1.0, recon: Optional[bytearray] = None) -> ClipStats:
+    """evalClip of one clip; `recon`, when given, must hold frames*w*h*3/2
+    bytes and receives the decoded clip."""
     st = ClipStats()
-    rc = lib().csc_eval_clip(data, w, h, frames, C.byref(g), flags, C.byref(st))
+    if recon is None and qscale == 1.0:
+        rc = lib().csc_eval_clip(data, w, h, frames, C.byref(g), flags, C.byref(st))
+    else:
+        buf = None
+        if recon is not None:
+            if len(recon) < frames * w * h * 3 // 2:
+                raise ValueError("recon buffer smaller than the clip")
+            buf = (C.c_uint8 * len(recon)).from_buffer(recon)
+        rc = lib().csc_eval_clip_ex(data, w, h, frames, C.byref(g), flags, qscale, buf, C.byref(st))
     if rc < 0:
         raise CodecError(rc, f"genome {g.id}")
     return st
@@ -125,12 +137,14 @@ def _aggregate(stats: List[ClipStats], clip_px: List[int]) -> Dict[str, float]:
         m = min(max(f32(st.msssim_y), f32(1e-6)), f32(1.0 - 1e-9))
         acc["ms_db"] = f32(acc["ms_db"] + f32(f32(-10.0) * np.log10(f32(f32(1.0) - m))))
         acc["dec"] = f32(acc["dec"] + f32(st.dec_mops))
+        acc["enc"] = f32(acc["enc"] + f32(st.enc_ms))
     tot_bytes = sum(int(st.bytes) for st in stats)
     bpp = float(tot_bytes) * 8.0 / float(sum(clip_px))
     psnr = float(f32(acc["psnr"] / n))
     return {"psnr": psnr, "bpp": bpp, "dec_mops": float(acc["dec"]), "psnr_u": float(f32(acc["psnr_u"] / n)),
             "psnr_v": float(f32(acc["psnr_v"] / n)), "msssim": float(f32(acc["msssim"] / n)),
-            "msssim_db": float(f32(acc["ms_db"] / n)), "bytes": tot_bytes, "label": psnr - LAMBDA * bpp}
+            "msssim_db": float(f32(acc["ms_db"] / n)), "enc_ms10": float(f32(acc["enc"] * f32(10.0))),
+            "bytes": tot_bytes, "label": psnr - LAMBDA * bpp}
 
 
 class Evaluator:

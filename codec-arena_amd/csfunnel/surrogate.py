@@ -11,7 +11,7 @@ in the reference.
 from __future__ import annotations
 
 import time
-from typing import Dict, List, Optional, Sequence
+from typing import Callable, Dict, List, Optional
 
 import numpy as np
 import torch
@@ -95,7 +95,8 @@ def _next_elites(surv, gen: int):
 
 def funnel_compare(dev: Screen, elites, m: MLP, w: np.ndarray, evaluator, gens: int = 2,
                    lam: int = 2_650_000_000, keep: int = 64, threads: int = 4, scorer: int = SCORER_MLP_BF16,
-                   sigma0: float = 0.12, jump: float = 0.01, imm: float = 0.01, pool: int = 1024) -> Dict[str, object]:
+                   sigma0: float = 0.12, jump: float = 0.01, imm: float = 0.01, pool: int = 1024,
+                   progress: Optional[Callable[[Dict[str, float]], None]] = None) -> Dict[str, object]:
     """Three screens from the same elites for `gens` chained generations
     (survivors become that screen's next elites, genes 16..28 reset to the
     legacy values); each generation's `keep` survivors are re-scored by the
@@ -105,7 +106,10 @@ def funnel_compare(dev: Screen, elites, m: MLP, w: np.ndarray, evaluator, gens: 
       ensemble  the BF16 MLP screen keeps `pool` on the device, re-ranked by
                 the mean of the FP64 MLP and quadratic predictions (both
                 predict the label).
-    Returns per-generation oracle statistics and timings."""
+    Returns per-generation oracle statistics and timings, and per arm the
+    mean over generations of the survivors' mean label, the best label seen
+    and the last generation's mean.  `progress` is called with each
+    generation's record as it completes."""
     qm = quad_model(w)
     arms = ("quad", "mlp", "ensemble")
     el = {a: elites for a in arms}
@@ -139,4 +143,10 @@ def funnel_compare(dev: Screen, elites, m: MLP, w: np.ndarray, evaluator, gens: 
             rec[f"{name}_top8_mean"] = float(np.sort(lab)[-8:].mean())
             el[name] = _next_elites(surv, gen)
         out.append(rec)
-    return {"elites_mean": float(base.mean()), "elites_max": float(base.max()), "generations": out}
+        if progress is not None:
+            progress(rec)
+    summary = {a: {"mean_of_means": float(np.mean([r[f"{a}_mean"] for r in out])),
+                   "best": float(max(r[f"{a}_max"] for r in out)),
+                   "last_mean": out[-1][f"{a}_mean"]} for a in arms}
+    return {"elites_mean": float(base.mean()), "elites_max": float(base.max()), "generations": out,
+            "summary": summary}

@@ -32,6 +32,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "cscodec.h"
 
@@ -84,6 +85,7 @@ typedef struct pheno {
     int cfl;          /* g[19] > 0.5: chroma-from-luma modes join the chroma set */
     int cdef;         /* g[20] > 0.5: CDEF-lite after deblocking */
     int qpel;         /* g[21] > 0.5: quarter-pel MC supersedes the bilinear half-pel path */
+    float qscale;     /* uniform quantizer override (src/codec.zig:750), 1 = off */
 } pheno;
 
 static void phenotype(const cs_genome* g, pheno* p) {
@@ -110,6 +112,7 @@ static void phenotype(const cs_genome* g, pheno* p) {
     p->cfl = clamp01(g->g[19]) > 0.5f;
     p->cdef = clamp01(g->g[20]) > 0.5f;
     p->qpel = g->g[21] > 0.5f;
+    p->qscale = 1.0f;
 }
 
 int csc_genome_supported(const cs_genome* g) {
@@ -138,7 +141,7 @@ static void band_steps(const pheno* p, float qp, float base_scale, const regctx*
     for (int b = 0; b < 3; b++) {
         float s = base_scale * m * p->band[b];
         if (rc->intra && b == 0) s *= p->intra_b0;
-        q[b] = fmaxf(1.0f, qp * s);
+        q[b] = fmaxf(1.0f, qp * s * p->qscale);
     }
 }
 
@@ -1434,13 +1437,26 @@ int csc_plane_msssim(const uint8_t* src, const uint8_t* rec, uint32_t w, uint32_
 #define CLIP_STREAM_CAP (8u << 20) /* bitbuf of src/codec.zig:2563 */
 #define PLANE_SCRATCH_CAP (1u << 20)
 
+static double now_ms(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
+}
+
 int csc_eval_clip(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames, const cs_genome* g,
                   uint32_t flags, csc_clip_stats* out) {
+    return csc_eval_clip_ex(yuv, w, h, frames, g, flags, 1.0f, NULL, out);
+}
+
+int csc_eval_clip_ex(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames, const cs_genome* g,
+                     uint32_t flags, float qscale, uint8_t* recon, csc_clip_stats* out) {
     if (!yuv || !g || !out || w < 16 || h < 16 || (w % 16) || (h % 16) || !frames) return CSC_E_ARGS;
+    if (!isfinite(qscale)) return CSC_E_ARGS;
     if (!csc_genome_supported(g)) return CSC_E_UNSUPPORTED;
     pthread_once(&tables_once, init_tables);
     pheno p;
     phenotype(g, &p);
+    p.qscale = qscale;
     codec cx;
     memset(&cx, 0, sizeof cx);
     cx.p = &p;
@@ -1473,12 +1489,14 @@ int csc_eval_clip(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames, c
     cx.prev[2] = cx.prev[1] + cx.csz;
     cx.main = (bytes_out){stream, 0, CLIP_STREAM_CAP, 0};
     cx.scratch = (bytes_out){stream + CLIP_STREAM_CAP, 0, PLANE_SCRATCH_CAP, 0};
+    const double t_enc = now_ms();
     for (uint32_t fi = 0; fi < frames; fi++) {
         const size_t fb_start = cx.main.n;
         uint8_t* rec[3] = {encout + fi * fsz, encout + fi * fsz + cx.ysz, encout + fi * fsz + cx.ysz + cx.csz};
         encode_frame(&cx, yuv, fi, rec);
         cx.last_frame_bytes = cx.main.n - fb_start;
     }
+    const float enc_ms = (float)(now_ms() - t_enc);
     if (cx.main.overflow) {
         rc = CSC_E_OVERFLOW;
         goto done;
@@ -1531,6 +1549,15 @@ int csc_eval_clip(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames, c
             acc_ms += m;
         }
     }
+    if (recon) {
+        /* global-planar like the clip: all Y frames, then all U, then all V */
+        for (uint32_t fi = 0; fi < frames; fi++) {
+            const uint8_t* d = decout + fi * fsz;
+            memcpy(recon + (size_t)fi * cx.ysz, d, cx.ysz);
+            memcpy(recon + (size_t)frames * cx.ysz + (size_t)fi * cx.csz, d + cx.ysz, cx.csz);
+            memcpy(recon + (size_t)frames * (cx.ysz + cx.csz) + (size_t)fi * cx.csz, d + cx.ysz + cx.csz, cx.csz);
+        }
+    }
     const float nf = (float)frames;
     out->bytes = cx.main.n;
     out->psnr_y = acc / nf;
@@ -1541,6 +1568,7 @@ int csc_eval_clip(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames, c
     out->idct_calls = cx.idct_calls;
     out->enc_mops = (float)(enc_dct + cx.idct_calls) * 64.0f / 1e6f;
     out->dec_mops = (float)cx.idct_calls * 64.0f / 1e6f;
+    out->enc_ms = enc_ms;
 done:
     free(planes);
     free(stream);

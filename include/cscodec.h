@@ -46,7 +46,7 @@ typedef struct csc_clip_stats {
     float dec_mops;   /* idct calls * 64 / 1e6 (encoder and decoder calls) */
     uint64_t dct_calls, idct_calls;
     int32_t closed_loop; /* 1 when the decoder output equals the encoder recon */
-    int32_t _pad;
+    float enc_ms;        /* wall time of the encoder (ms); not reproducible */
 } csc_clip_stats;
 
 /* 1 when every tool the genome switches on is restated here, else 0. */
@@ -60,6 +60,14 @@ int csc_genome_supported(const cs_genome* g);
  * the encoder recon; that is reproduced and reported in closed_loop. */
 int csc_eval_clip(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames, const cs_genome* g,
                   uint32_t flags, csc_clip_stats* out);
+
+/* csc_eval_clip with the `arena eval` options of src/main.zig:74-93:
+ * `qscale` is the uniform quantizer override (--qscale, codec.qscale_override
+ * at src/codec.zig:750-754; 1 = off, bit-identical to csc_eval_clip), and a
+ * non-null `recon` (frames * w * h * 3 / 2 bytes) receives the decoded clip in
+ * the input's global-planar layout (--dump-recon, src/codec.zig:2649-2658). */
+int csc_eval_clip_ex(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames, const cs_genome* g,
+                     uint32_t flags, float qscale, uint8_t* recon, csc_clip_stats* out);
 
 /* planeMsssim (src/msssim.zig:89-175) of two u8 planes. */
 int csc_plane_msssim(const uint8_t* src, const uint8_t* rec, uint32_t w, uint32_t h, float* out);

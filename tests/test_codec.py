@@ -296,8 +296,124 @@ def test_codec_library_exports_every_declared_symbol(cc):
     txt = open(os.path.join(os.path.dirname(GOLDEN), "..", "include", "cscodec.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     names = sorted(set(re.findall(r"\b(csc_[a-z0-9_]+)\s*\(", txt)))
-    assert len(names) == 9
+    assert len(names) == 10
     L = cc.lib()
     for n in names:
         assert hasattr(L, n) and n in cc.SIGNATURES, n
     assert C.sizeof(cc.ClipStats) == 8 + 6 * 4 + 8 + 8 + 4 + 4
+
+
+# ------------------------------------------------------------ arena eval CLI
+
+def test_zig_decimal_matches_the_oracle_printer(oracle):
+    """{d:.4} of f32 against the oracle's printer (src/screen.zig:330 format),
+    including values below 1e-4 and round-half-up carries."""
+    from csfunnel.evalcmd import zig_decimal
+    rng = np.random.default_rng(11)
+    vals = np.concatenate([rng.random(2000), rng.random(500) * 1e-3, rng.random(300) * 1e4,
+                           -rng.random(200), [0.0, 0.99996, 0.99995, 9.99995, 5e-05, 4e-05, 1e-45]])
+    for v in vals.astype(np.float32):
+        assert zig_decimal(float(v), 4) == oracle.fmt(float(v)), repr(v)
+
+
+@pytest.mark.parametrize("v,prec,bits,s", [(31.675, 2, 32, "31.68"), (0.18155, 4, 64, "0.1816"),
+                                           (1784.25, 1, 32, "1784.3"), (0.0, 2, 32, "0.00"),
+                                           (0.828749, 5, 32, "0.82875"), (12.5, 6, 32, "12.500000"),
+                                           (0.99999, 2, 32, "1.00"), (-3.14159, 2, 64, "-3.14")])
+def test_zig_decimal_known(v, prec, bits, s):
+    from csfunnel.evalcmd import zig_decimal
+    assert zig_decimal(v, prec, bits) == s
+
+
+def _corpus_subset(clips, d, names):
+    os.makedirs(d, exist_ok=True)
+    for n in names:
+        dst = os.path.join(d, n + ".yuv")
+        if not os.path.exists(dst):
+            os.symlink(os.path.join(clips, n + ".yuv"), dst)
+    with open(os.path.join(d, "corpus.txt"), "w") as f:
+        f.write("# name w h frames fps\n" + "".join(f"{n} 320 192 24 24\n" for n in names))
+    return d
+
+
+def test_arena_eval_writes_the_halloffame_rows(cc, clips, tmp_path, capfd):
+    """`arena eval` on the five legacy clips over a population file of all 22
+    hall-of-fame genomes (33-token v5 lines, "%.4f" genes) prints their
+    published PSNR, bpp, dec_mops, chroma PSNR, MS-SSIM and MS-SSIM dB
+    columns digit for digit; a 35-token line and a superblock genome are
+    counted bad, a 3-token line is skipped (src/main.zig:279-355)."""
+    from csfunnel.arena import main
+    kats = json.load(open(os.path.join(GOLDEN, "halloffame_kats.json")))
+    recs = kats["records"] + [kats["champion"]]
+    corp = _corpus_subset(clips, str(tmp_path / "corpus"), LEGACY5)
+    lines = [f"{r['id']} 0 0 0 0 " + " ".join("%.4f" % x for x in r["g"]) for r in recs]
+    sb16 = "99 0 0 0 0 " + " ".join(["0.5000"] * 22) + " 0.9000 " + " ".join(["0.5000"] * 6)
+    pop = tmp_path / "pop.txt"
+    pop.write_text("\n".join(lines[:3] + ["1 2 3", " ".join(["1"] * 35), sb16] + lines[3:]) + "\n")
+    out = tmp_path / "res.csv"
+    assert main(["eval", "--corpus", corp, "--pop", str(pop), "--out", str(out)]) == 0
+    err = capfd.readouterr().err
+    assert f"eval: {len(recs)} ok, 2 bad, 5 clips" in err and "[evalClip error: error.Unsupported clip=drift]" in err
+    rows = [ln.split(",") for ln in out.read_text().splitlines()]
+    assert [int(t[0]) for t in rows] == [r["id"] for r in recs]
+    cols = {"psnr": 1, "bpp": 2, "dec_mops": 4, "psnr_u": 5, "psnr_v": 6, "msssim": 7, "msssim_db": 8}
+    checked = 0
+    for r, t in zip(recs, rows):
+        assert len(t) == 9
+        for k, c in cols.items():
+            assert len(t[c].split(".")[1]) == PRINTED[k]
+            if k in r["metrics"]:
+                assert float(t[c]) == r["metrics"][k], (r["id"], k, t[c], r["metrics"][k])
+                checked += 1
+    assert checked >= 110
+
+
+def test_arena_eval_qscale_recon_and_psnr(cc, clips, tmp_path):
+    """--qscale 1 is the default codec; a coarser --qscale spends fewer bits;
+    --dump-recon writes the decoded clip, whose `arena psnr` against the
+    source is the eval's PSNR; --clips restricts and orders the clip set."""
+    from csfunnel.arena import main
+    kat = json.load(open(os.path.join(GOLDEN, "halloffame_kats.json")))["records"][0]
+    corp = _corpus_subset(clips, str(tmp_path / "corpus"), LEGACY5)
+    pop = tmp_path / "pop.txt"
+    pop.write_text(f"{kat['id']} 0 0 0 0 " + " ".join("%.4f" % x for x in kat["g"]) + "\n")
+    sub = tmp_path / "clips.txt"
+    sub.write_text("# subset\ngrid\nnot-a-clip\n")
+
+    def run(*extra):
+        out = tmp_path / "r.csv"
+        assert main(["eval", "--corpus", corp, "--pop", str(pop), "--out", str(out), "--clips", str(sub)]
+                    + list(extra)) == 0
+        t = out.read_text().split(",")
+        return [float(x) for x in t[1:3] + t[4:]]
+
+    base = run()
+    assert run("--qscale", "1.0") == base
+    coarse = run("--qscale", "2.0")
+    assert coarse[1] < base[1] and coarse[0] < base[0]
+    rd = tmp_path / "recon"
+    assert run("--dump-recon", str(rd)) == base
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(os.path.dirname(GOLDEN), "..", "codec-arena_amd", "bin", "arena"),
+                        "psnr", os.path.join(corp, "grid.yuv"), str(rd / "grid.recon.yuv"), "320", "192", "24"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    y, u, v = (float(x) for x in r.stdout.split())
+    assert abs(y - base[0]) <= 0.005 and abs(u - base[3]) <= 0.005 and abs(v - base[4]) <= 0.005
+
+
+def test_arena_eval_errors(cc, clips, tmp_path, capfd):
+    from csfunnel.arena import main
+    assert main(["eval", "--corpus", str(tmp_path / "none"), "--pop", "x", "--out", str(tmp_path / "o")]) == 1
+    assert "error: FileNotFound" in capfd.readouterr().err
+    corp = _corpus_subset(clips, str(tmp_path / "corpus"), ("drift",))
+    sub = tmp_path / "clips.txt"
+    sub.write_text("nothing\n")
+    assert main(["eval", "--corpus", corp, "--pop", "x", "--clips", str(sub)]) == 1
+    assert "error: NoClipsMatched" in capfd.readouterr().err
+    with open(os.path.join(corp, "corpus.txt"), "a") as f:
+        f.write("short 320 192\n")
+    assert main(["eval", "--corpus", corp, "--pop", "x"]) == 1
+    assert "error: BadSpec" in capfd.readouterr().err
+    assert main(["frobnicate"]) == 1 and main([]) == 1
