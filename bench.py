@@ -45,6 +45,36 @@ PUBLISHED_CPU = 17.2e6    # reference Zig SIMD screen, 4 threads (docs/SCALE.md:
 VALU_LANE_INSTR = 2222
 VALU_PEAK = 256 * 128 * 2.4e9  # lane-instructions/s: 256 CUs x 128 lanes/cycle x 2.4 GHz
 SCORE_TRAFFIC_PER_LAUNCH = 175.1e6 / 11
+# Multiply-class and 64-bit-add instructions among the 2222 of one candidate:
+# counted in the ISA of the compiled k_score (quadratic scorer, fast
+# mutation), in the two straight-line blocks of a normal candidate (parent
+# picks, crossover bits and gene loads; mutation draws, mutation and scoring).
+VALU_CLASS_PER_CANDIDATE = {"v_mad_u64_u32": 97, "v_mul_lo_u32": 223, "v_mul_hi_u32": 36, "v_lshl_add_u64": 91}
+
+
+def valu_model(dev_rates, cand_per_s):
+    """The screen's VALU issue rate against the rate the same device sustains
+    for a plain stream of 32-bit adds at full occupancy (the probe), and the
+    probe's cost of the screen's multiply and 64-bit classes relative to an
+    add."""
+    ns = dev_rates["ns_per_instr"]
+    add_rate = dev_rates["simds"] * 64 / (ns["v_add_u32"] * 1e-9)  # lane-instructions/s
+    achieved = cand_per_s * VALU_LANE_INSTR
+    return {"probe_add_lane_instr_per_s": add_rate, "achieved_lane_instr_per_s": achieved,
+            "frac_of_add_stream": achieved / add_rate,
+            "instr_per_candidate": dict(VALU_CLASS_PER_CANDIDATE, total=VALU_LANE_INSTR),
+            "cost_vs_add": {c: round(ns[c] / ns["v_add_u32"], 3) for c in VALU_CLASS_PER_CANDIDATE}}
+
+
+def mfma_model(dev_rates, flop_per_s):
+    """The MLP screen's MFMA rate against the probe's v_mfma_f32_16x16x32_bf16
+    stream, and whether packed-f16 work issued beside the MFMAs is hidden
+    (cost of MFMA + 4 v_pk_fma_f16 against MFMA alone and their sum)."""
+    ns = dev_rates["ns_per_instr"]
+    probe = dev_rates["simds"] * 16384 / (ns["v_mfma_f32_16x16x32_bf16"] * 1e-9) / 1e12
+    return {"probe_mfma_tflops": probe, "frac_of_probe": flop_per_s / 1e12 / probe,
+            "ns_mfma": ns["v_mfma_f32_16x16x32_bf16"], "ns_mfma_plus_4pk": ns["mfma+4pk_fma_f16"],
+            "ns_4pk": 4 * ns["v_pk_fma_f16"]}
 
 
 def sigma_of(g):
@@ -162,7 +192,7 @@ def run_generations(dev, model, elites, scorer, mlp, steps, warmup, keep):
     return time.perf_counter() - t0, stats
 
 
-def mlp_line(scorer, steps, dt, stats, train_info):
+def mlp_line(scorer, steps, dt, stats, train_info, rates=None):
     ms_score = sum(s["ms_score"] for s in stats)
     launches = sum(s["levels"] for s in stats)
     achieved = LAMBDA * steps * MLP_FLOP_PER_CANDIDATE / (ms_score * 1e-3) / 1e12
@@ -171,7 +201,8 @@ def mlp_line(scorer, steps, dt, stats, train_info):
             "roofline": {"bound": "mfma", "kernel": "k_score<mlp>", "achieved": achieved, "peak": PEAK_MFMA[scorer],
                          "unit": "TFLOP/s", "frac": achieved / PEAK_MFMA[scorer],
                          "flop_per_candidate": MLP_FLOP_PER_CANDIDATE, "launches": launches,
-                         "avg_launch_ms": ms_score / max(1, launches), "k_score_ms_per_step": ms_score / steps},
+                         "avg_launch_ms": ms_score / max(1, launches), "k_score_ms_per_step": ms_score / steps,
+                         **({"probe": mfma_model(rates, achieved * 1e12)} if rates and scorer == 1 else {})},
             "surrogate": {"train_pairs": 20000, "epochs": 300, "train_s": round(train_info["train_s"], 2),
                           "final_loss": float(train_info["final_loss"]),
                           "labels": "PSNR - 14 bpp from the codec label source (ten-clip corpus)",
@@ -235,8 +266,9 @@ def main():
             train_info.pop("genes")
         dt, stats = run_generations(dev, model, elites, scorer, mlp, args.steps, args.warmup, 64)
         raw, rr = mlp_recall(dev, elites, scorer, mlp)
+        rates = scr.valu_rates(0)
         dev.close()
-        ml = mlp_line(scorer, args.steps, dt, stats, train_info)
+        ml = mlp_line(scorer, args.steps, dt, stats, train_info, rates)
         out = {"metric": "candidates screened/s on 1 GPU (learned surrogate, %s)" % args.scorer,
                "value": ml.pop("value"), "unit": ml.pop("unit"), "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": ml.pop("ms_per_step"), "higher_is_better": True,
@@ -250,6 +282,7 @@ def main():
 
     dt, stats = run_generations(dev, model, elites, 0, None, args.steps, args.warmup, KEEP)
     value = LAMBDA * args.steps / dt
+    rates = scr.valu_rates(0)
     recall = {"quad_fp32": quad_recall(dev, model, elites), "k_quad": KEEP, "lambda": RECALL_LAMBDA}
     mlp_out = None
     funnel = None
@@ -259,7 +292,7 @@ def main():
         mdt, mstats = run_generations(dev, model, elites, 1, mlp, 2, 1, 64)
         raw, rr = mlp_recall(dev, elites, 1, mlp)
         recall.update({"mlp_bf16": raw, "mlp_bf16_after_rerank": rr, "k_mlp": 64})
-        mlp_out = mlp_line(1, 2, mdt, mstats, train_info)
+        mlp_out = mlp_line(1, 2, mdt, mstats, train_info, rates)
         if args.funnel_gens > 0:
             from csfunnel import codec, surrogate
             print("funnel: quadratic vs MLP screen, survivors re-scored by the codec label source", file=sys.stderr,
@@ -296,7 +329,8 @@ def main():
                      "valu_issue": {"lane_instr_per_candidate": VALU_LANE_INSTR,
                                     "achieved": LAMBDA * args.steps / (ms_score * 1e-3) * VALU_LANE_INSTR,
                                     "peak": VALU_PEAK, "unit": "lane-instr/s",
-                                    "frac": LAMBDA * args.steps / (ms_score * 1e-3) * VALU_LANE_INSTR / VALU_PEAK},
+                                    "frac": LAMBDA * args.steps / (ms_score * 1e-3) * VALU_LANE_INSTR / VALU_PEAK,
+                                    "probe": valu_model(rates, LAMBDA * args.steps / (ms_score * 1e-3))},
                      "flop_per_candidate": FLOP_PER_CANDIDATE, "launches": launches,
                      "avg_launch_ms": ms_score / max(1, launches), "k_map_ms_per_step": ms_map / args.steps,
                      "k_score_ms_per_step": ms_score / args.steps},

@@ -96,6 +96,8 @@ SIGNATURES = {
     "cs_format_gene": (C.c_int, [C.c_float, C.c_char_p, C.c_size_t]),
     "cs_write_promoted": (C.c_int, [C.c_char_p, C.POINTER(Survivor), C.c_uint64]),
     "cs_write_count": (C.c_int, [C.c_char_p, C.c_uint64]),
+    "cs_valu_rate_probe": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_uint32)]),
 }
 
 _lib = None

@@ -21,7 +21,7 @@ REPO = os.path.dirname(PKG_ROOT)
 LIB = os.path.join(HERE, "libcsfunnel.so")
 CODEC_LIB = os.path.join(HERE, "libcscodec.so")
 
-SOURCES = ["screen_device.hip", "ridge_device.hip", "mlp_device.hip", "funnel_host.cpp"]
+SOURCES = ["screen_device.hip", "ridge_device.hip", "mlp_device.hip", "rate_probe.hip", "funnel_host.cpp"]
 ARCH = "gfx950"
 
 

@@ -188,6 +188,26 @@ def write_count(path: str, total: int) -> None:
     check(lib().cs_write_count(path.encode(), int(total)))
 
 
+VALU_CLASSES = ("v_add_u32", "v_mul_lo_u32", "v_mul_hi_u32", "v_mad_u64_u32", "v_mul_f32", "v_lshl_add_u64",
+                "v_alignbit_b32", "v_pk_fma_f16", "v_mfma_f32_16x16x32_bf16", "mfma+4pk_fma_f16")
+
+
+def valu_rates(device: int = 0, iters: int = 1 << 15) -> dict:
+    """Measured issue cost of each instruction class at full occupancy: ns
+    and shader-clock cycles per instruction per SIMD (the last class: per
+    MFMA, with four independent v_pk_fma_f16 issued beside each), the clock
+    those imply, and the SIMD count."""
+    ns_out, cyc_out = {}, {}
+    simds = C.c_uint32(0)
+    for op, name in enumerate(VALU_CLASSES):
+        ns, cyc = C.c_double(0.0), C.c_double(0.0)
+        check(lib().cs_valu_rate_probe(device, op, iters, C.byref(ns), C.byref(cyc), C.byref(simds)))
+        ns_out[name] = ns.value / 8.0
+        cyc_out[name] = cyc.value / 8.0
+    ghz = sum(cyc_out.values()) / sum(ns_out.values())
+    return {"ns_per_instr": ns_out, "cycles_per_instr": cyc_out, "clock_ghz": ghz, "simds": simds.value}
+
+
 def format_gene(v: float) -> str:
     buf = C.create_string_buffer(64)
     lib().cs_format_gene(C.c_float(v), buf, 64)

@@ -59,19 +59,26 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 constexpr float GELU_Q = 0.353515625f;  // f16(sqrt 2) / 4, exact in f16
 __device__ __forceinline__ h2 h2s(_Float16 c) { return h2{c, c}; }
 __device__ __forceinline__ h2 pk_rtz(float a, float b) { return __builtin_bit_cast(h2, __builtin_amdgcn_cvt_pkrtz(a, b)); }
-__device__ __forceinline__ h2 gelu_sig_h2(h2 x, h2 q) {
-    const h2 v = x * q;
-    const h2 w = __builtin_elementwise_min(__builtin_elementwise_fma(v, v, h2s(-1.0f16)), h2s(1.0f16));
-    h2 p = h2s(0.022354126f16);
-    p = __builtin_elementwise_fma(p, w, h2s(-0.0514831543f16));
-    p = __builtin_elementwise_fma(p, w, h2s(0.0627441406f16));
-    p = __builtin_elementwise_fma(p, w, h2s(-0.0956420898f16));
-    p = __builtin_elementwise_fma(p, w, h2s(0.157470703f16));
-    p = __builtin_elementwise_fma(p, w, h2s(-0.239624023f16));
-    p = __builtin_elementwise_fma(p, w, h2s(0.497802734f16));
-    const h2 sg = __builtin_elementwise_fma(v, p, h2s(0.5f16));
-    return __builtin_elementwise_min(__builtin_elementwise_max(sg, h2s(0.0f16)), h2s(1.0f16));
+// V is a vector of f16 (2 or 8 lanes): an 8-lane vector is four independent
+// packed chains, so each Horner step issues four v_pk_fma_f16 back to back
+// instead of waiting on one dependent chain.
+template <typename V>
+__device__ __forceinline__ V gelu_sig_v(V x, V q) {
+    const V one = (V)(_Float16)1.0f;
+    const V v = x * q;
+    const V w = __builtin_elementwise_min(__builtin_elementwise_fma(v, v, -one), one);
+    V p = (V)(_Float16)0.022354126f;
+    p = __builtin_elementwise_fma(p, w, (V)(_Float16)-0.0514831543f);
+    p = __builtin_elementwise_fma(p, w, (V)(_Float16)0.0627441406f);
+    p = __builtin_elementwise_fma(p, w, (V)(_Float16)-0.0956420898f);
+    p = __builtin_elementwise_fma(p, w, (V)(_Float16)0.157470703f);
+    p = __builtin_elementwise_fma(p, w, (V)(_Float16)-0.239624023f);
+    p = __builtin_elementwise_fma(p, w, (V)(_Float16)0.497802734f);
+    const V sg = __builtin_elementwise_fma(v, p, (V)(_Float16)0.5f);
+    return __builtin_elementwise_min(__builtin_elementwise_max(sg, (V)(_Float16)0.0f), one);
 }
+__device__ __forceinline__ h2 gelu_sig_h2(h2 x, h2 q) { return gelu_sig_v<h2>(x, q); }
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
 // Layouts prepared for the screen (one wavefront scores 64 candidates):
 //   w1f  [16 m-tiles][64 lanes] x 4 bf16: A fragments of W1^T for

@@ -21,6 +21,8 @@ namespace csf {
 
 // ------------------------------------------------------------------ GEMM
 
+constexpr int KSPLIT = 16;  // K layers of the weight-gradient GEMMs
+
 struct GemmArgs {
     int M, N, K;
     const __bf16 *A;
@@ -33,10 +35,16 @@ struct GemmArgs {
     __bf16 *Cb;         // output (bf16) or null
     int ldc;
     int act;  // 0 identity, 1 gelu
+    int kchunk;         // split-K: K per workgroup layer (blockIdx.z), 0 = all of K
+    long split_stride;  // split-K: elements between the partial outputs of consecutive layers
 };
 
 // 64 x 64 output tile per workgroup, 2 x 2 waves of 32 x 32 (2 x 2 MFMA tiles);
-// K advances 32 per step through LDS (B stored transposed, [n][k]).
+// K advances 32 per step through LDS (B stored transposed, [n][k]).  With
+// kchunk > 0 the workgroups of layer z = blockIdx.z cover K range
+// [z kchunk, (z + 1) kchunk) and write their partial tile to C + z split_stride
+// (the weight gradients, whose K is the batch and whose output is one or a few
+// tiles, would otherwise run on a handful of workgroups).
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     __shared__ __bf16 As[64][40];
     __shared__ __bf16 Bs[64][40];
@@ -48,13 +56,15 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.K; k0 += 32) {
+    const int kbeg = g.kchunk ? (int)blockIdx.z * g.kchunk : 0;
+    const int kend = g.kchunk ? min(g.K, kbeg + g.kchunk) : g.K;
+    for (int k0 = kbeg; k0 < kend; k0 += 32) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int idx = tid * 8 + e, r = idx >> 5, kk = idx & 31;
             const int gm = tm + r, gn = tn + r, gk = k0 + kk;
-            As[r][kk] = (gm < g.M && gk < g.K) ? g.A[(long)gm * g.lda_m + (long)gk * g.lda_k] : (__bf16)0.0f;
-            Bs[r][kk] = (gn < g.N && gk < g.K) ? g.B[(long)gk * g.ldb_k + (long)gn * g.ldb_n] : (__bf16)0.0f;
+            As[r][kk] = (gm < g.M && gk < kend) ? g.A[(long)gm * g.lda_m + (long)gk * g.lda_k] : (__bf16)0.0f;
+            Bs[r][kk] = (gn < g.N && gk < kend) ? g.B[(long)gk * g.ldb_k + (long)gn * g.ldb_n] : (__bf16)0.0f;
         }
         __syncthreads();
 #pragma unroll
@@ -78,13 +88,24 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
                 const int col = tn + wn * 32 + ni * 16 + (lane & 15);
                 if (row < g.M && col < g.N) {
                     float z = acc[mi][ni][r] + (g.bias ? g.bias[col] : 0.0f);
-                    const long o = (long)row * g.ldc + col;
+                    const long o = (long)row * g.ldc + col + (long)blockIdx.z * g.split_stride;
                     if (g.Z) g.Z[o] = z;
                     const float h = g.act ? gelu_f(z) : z;
                     if (g.C) g.C[o] = h;
                     if (g.Cb) g.Cb[o] = (__bf16)h;
                 }
             }
+}
+
+// C[row][col] = sum over z = 0..nz-1 of part[z][row][col] (fixed order)
+__global__ void k_sum_splits(const float *__restrict__ part, int nz, int M, int N, int ldc, long stride,
+                             float *__restrict__ C) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)M * N) return;
+    const long o = (i / N) * ldc + (i % N);
+    float s = 0.0f;
+    for (int z = 0; z < nz; ++z) s += part[z * stride + o];
+    C[o] = s;
 }
 
 // ------------------------------------------------------------ elementwise
@@ -224,7 +245,8 @@ static int mlp_alloc(cs_mlp *m, int n) {
         (rc = ensure(m->h2b, B * MLP_H * 2)) || (rc = ensure(m->out, B * 4)) || (rc = ensure(m->dout, B * 4)) ||
         (rc = ensure(m->dz2f, B * MLP_H * 4)) || (rc = ensure(m->dz2b, B * MLP_H * 2)) ||
         (rc = ensure(m->dh1, B * MLP_H * 4)) || (rc = ensure(m->dz1f, B * MLP_H * 4)) ||
-        (rc = ensure(m->dz1b, B * MLP_H * 2)) || (rc = ensure(m->loss_part, (B / 256 + 2) * 4)))
+        (rc = ensure(m->dz1b, B * MLP_H * 2)) || (rc = ensure(m->loss_part, (B / 256 + 2) * 4)) ||
+        (rc = ensure(m->gws, (size_t)KSPLIT * MLP_H * MLP_H * 4)))
         return rc;
     m->cap = n;
     return CS_OK;
@@ -232,8 +254,20 @@ static int mlp_alloc(cs_mlp *m, int n) {
 
 static void launch_gemm(hipStream_t s, int M, int N, int K, const __bf16 *A, long lam, long lak, const __bf16 *B,
                         long lbk, long lbn, const float *bias, float *Z, float *C, __bf16 *Cb, int act) {
-    GemmArgs g{M, N, K, A, lam, lak, B, lbk, lbn, bias, Z, C, Cb, MLP_H, act};
+    GemmArgs g{M, N, K, A, lam, lak, B, lbk, lbn, bias, Z, C, Cb, MLP_H, act, 0, 0};
     hipLaunchKernelGGL(k_gemm, dim3((N + 63) / 64, (M + 63) / 64), dim3(256), 0, s, g);
+}
+
+// weight gradient C = A^T B over the batch (K = n): KSPLIT layers of K into
+// the workspace, then a fixed-order sum into C (row pitch MLP_H)
+static void launch_gemm_wgrad(hipStream_t s, int M, int N, int K, const __bf16 *A, long lak, const __bf16 *B,
+                              long lbk, float *ws, float *C) {
+    const int kchunk = ((K + KSPLIT * 32 - 1) / (KSPLIT * 32)) * 32;
+    const long stride = (long)M * MLP_H;
+    GemmArgs g{M, N, K, A, 1, lak, B, lbk, 1, nullptr, nullptr, ws, nullptr, MLP_H, 0, kchunk, stride};
+    hipLaunchKernelGGL(k_gemm, dim3((N + 63) / 64, (M + 63) / 64, KSPLIT), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)(((long)M * N + 255) / 256)), dim3(256), 0, s, (const float *)ws,
+                       KSPLIT, M, N, MLP_H, stride, C);
 }
 
 static int refresh(cs_mlp *m) {
@@ -275,8 +309,8 @@ static int backward(cs_mlp *m, int n) {
                        (__bf16 *)m->dz2b.p);
     hipLaunchKernelGGL(k_colsum, dim3(MLP_H), dim3(256), 0, s, (const float *)m->dz2f.p, n, g + P_B2);
     // dW2 = H1^T dZ2
-    launch_gemm(s, MLP_H, MLP_H, n, (const __bf16 *)m->h1b.p, 1, MLP_H, (const __bf16 *)m->dz2b.p, MLP_H, 1, nullptr,
-                nullptr, g + P_W2, nullptr, 0);
+    launch_gemm_wgrad(s, MLP_H, MLP_H, n, (const __bf16 *)m->h1b.p, MLP_H, (const __bf16 *)m->dz2b.p, MLP_H,
+                      (float *)m->gws.p, g + P_W2);
     // dH1 = dZ2 W2^T
     launch_gemm(s, n, MLP_H, MLP_H, (const __bf16 *)m->dz2b.p, MLP_H, 1, (const __bf16 *)m->w2b.p, 1, MLP_H, nullptr,
                 nullptr, (float *)m->dh1.p, nullptr, 0);
@@ -285,8 +319,8 @@ static int backward(cs_mlp *m, int n) {
                        (float *)m->dz1f.p, (__bf16 *)m->dz1b.p);
     hipLaunchKernelGGL(k_colsum, dim3(MLP_H), dim3(256), 0, s, (const float *)m->dz1f.p, n, g + P_B1);
     // dW1 = X^T dZ1
-    launch_gemm(s, MLP_IN, MLP_H, n, (const __bf16 *)m->xb.p, 1, MLP_IN, (const __bf16 *)m->dz1b.p, MLP_H, 1, nullptr,
-                nullptr, g + P_W1, nullptr, 0);
+    launch_gemm_wgrad(s, MLP_IN, MLP_H, n, (const __bf16 *)m->xb.p, MLP_IN, (const __bf16 *)m->dz1b.p, MLP_H,
+                      (float *)m->gws.p, g + P_W1);
     HIPCHK(hipGetLastError());
     return CS_OK;
 }
@@ -373,7 +407,7 @@ int cs_mlp_destroy(cs_mlp *m) {
     (void)hipStreamSynchronize(m->ctx->stream);
     csf::DevBuf *bufs[] = {&m->params, &m->grads, &m->m, &m->v, &m->w1b, &m->w2b, &m->w1f, &m->w1fh, &m->w2f, &m->w2f8,
                            &m->xb, &m->z1, &m->h1b, &m->z2, &m->h2b, &m->out, &m->dout, &m->dz2f, &m->dz2b,
-                           &m->dh1, &m->dz1f, &m->dz1b, &m->loss_part};
+                           &m->dh1, &m->dz1f, &m->dz1b, &m->loss_part, &m->gws};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
     m->ctx->n_mlp--;

@@ -418,7 +418,7 @@ struct MlpLds {
     const void *w2;        // W2 fragments (bf16 or fp8)
     const bf16x4 *w1f;     // W1^T fragments, global (fp8 screen: times h_scale)
     const float *b1, *b2;  // biases (fp8 screen: b1 h_scale, b2 h_scale w_scale)
-    const uint32_t *w3h;   // w3 as f16 pairs {w3[32 p + j], w3[32 p + 16 + j]} at 16 p + j
+    const uint32_t *w3h;   // w3 as f16 pairs, [n-tile pair][lane group][4]
     __bf16 *xs;
     float *sc;
 };
@@ -427,15 +427,15 @@ struct MlpLds {
 // Layer 1 computes H1^T = W1^T X^T on v_mfma_f32_16x16x16_bf16, so each
 // lane's accumulators are, for candidate l%16 of a group of 16, the hidden
 // units 16 mt + 4 (l/16) + r; packed two tiles at a time they are exactly the
-// A fragments of layer 2 in the permuted K order of w2_k_of().  Layer 2 runs
-// on v_mfma_f32_16x16x32_{bf16,fp8} over pairs of n-tiles, so that a lane
-// holds z2 of two hidden units for the same candidate: GELU runs on that f16
-// pair and layer 3 is one v_dot2c_f32_f16 with the matching w3 pair.
+// H1^T fragments of layer 2 in the permuted K order of w2_k_of().  Layer 2
+// runs on v_mfma_f32_16x16x32_{bf16,fp8} over pairs of n-tiles, so that a
+// lane holds z2 of eight hidden units of one candidate: GELU runs on them as
+// four f16 pairs and layer 3 is four v_dot2c_f32_f16 with the matching w3
+// pairs.
 template <int SC>
 __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml, int lane) {
     const int q = lane >> 4, j = lane & 15;
     typedef typename std::conditional<SC == SC_MLP_FP8, uint64_t, bf16x8>::type Frag;
-    const h2 q1 = h2s((_Float16)w.q1), q2 = h2s((_Float16)GELU_Q);
     // BF16: W1 fragments are fetched afresh for every batch (an opaque copy
     // of the pointer): held across batches they would be 32 more live
     // registers in the scoring loop and get spilled to scratch, whose reloads
@@ -447,17 +447,36 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
     bf16x4 w1v[16];
 #pragma unroll
     for (int mt = 0; mt < 16; ++mt) w1v[mt] = w1f[mt * 64 + lane];
+    const h8 q1v = (h8)(_Float16)w.q1, q2v = (h8)(_Float16)GELU_Q;
     Frag af[4][8];
 #pragma unroll
     for (int c4 = 0; c4 < 4; ++c4) {
         const s16x4 bx = *reinterpret_cast<const s16x4 *>(ml.xs + (c4 * 16 + j) * MLP_IN + 4 * q);
 #pragma unroll
-        for (int mt = 0; mt < 16; ++mt) {
-            f32x4 acc = *reinterpret_cast<const f32x4 *>(ml.b1 + 16 * mt + 4 * q);
-            const bf16x4 wa = w1v[mt];
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const s16x4 *>(&wa), bx, acc, 0, 0, 0);
-            const h2 x01 = pk_rtz(acc[0], acc[1]), x23 = pk_rtz(acc[2], acc[3]);
-            const h2 g01 = x01 * gelu_sig_h2(x01, q1), g23 = x23 * gelu_sig_h2(x23, q1);
+        for (int m4 = 0; m4 < 16; m4 += 4) {
+            // four m-tiles, then their eight GELU pairs as two 8-wide vectors
+            f32x4 acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                acc[t] = *reinterpret_cast<const f32x4 *>(ml.b1 + 16 * (m4 + t) + 4 * q);
+                const bf16x4 wa = w1v[m4 + t];
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(*reinterpret_cast<const s16x4 *>(&wa), bx, acc[t],
+                                                                   0, 0, 0);
+            }
+            h8 xa, xb;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const h2 x01 = pk_rtz(acc[t][0], acc[t][1]), x23 = pk_rtz(acc[t][2], acc[t][3]);
+                xa[2 * t] = x01[0];
+                xa[2 * t + 1] = x01[1];
+                xb[2 * t] = x23[0];
+                xb[2 * t + 1] = x23[1];
+            }
+            const h8 ga = xa * gelu_sig_v<h8>(xa, q1v), gb = xb * gelu_sig_v<h8>(xb, q1v);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+            const int mt = m4 + t;
+            const h2 g01 = h2{ga[2 * t], ga[2 * t + 1]}, g23 = h2{gb[2 * t], gb[2 * t + 1]};
             if constexpr (SC == SC_MLP_FP8) {
                 typedef short s16x2 __attribute__((ext_vector_type(2)));
                 s16x2 pk = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(s16x2{0, 0}, g01, 1.0f, false);
@@ -473,20 +492,26 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
                 af[c4][mt >> 1][(mt & 1) * 4 + 2] = b23[0];
                 af[c4][mt >> 1][(mt & 1) * 4 + 3] = b23[1];
             }
+            }
         }
     }
-    float part[4][4];
-#pragma unroll
-    for (int c4 = 0; c4 < 4; ++c4)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) part[c4][r] = 0.0f;
-    for (int np = 0; np < 8; ++np) {
-        const float ba = ml.b2[32 * np + j], bb = ml.b2[32 * np + 16 + j];
-        f32x4 ca[4], cb[4];
+    // Layer 2 runs transposed, Z2^T = W2^T H1^T: the W2 fragments are the A
+    // operands and the H1 fragments the B operands (both fragment layouts
+    // are lane -> row/column l%16, k-group l/16), so a lane's accumulators are
+    // four consecutive hidden units of one candidate and its layer-3 partial
+    // sum is a single register per group of 16 candidates.
+    float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    // The n-tile pairs are software-pipelined over two accumulator sets: the
+    // MFMAs of pair np + 1 sit in the same block as the GELU / layer-3
+    // epilogue of pair np, so the VALU work of one hides under the matrix
+    // work of the other inside one wave.
+    auto mfma_pair = [&](int np, f32x4 (&ca)[4], f32x4 (&cb)[4]) {
+        const f32x4 ba = *reinterpret_cast<const f32x4 *>(ml.b2 + 32 * np + 4 * q);
+        const f32x4 bb = *reinterpret_cast<const f32x4 *>(ml.b2 + 32 * np + 16 + 4 * q);
 #pragma unroll
         for (int c4 = 0; c4 < 4; ++c4) {
-            ca[c4] = f32x4{ba, ba, ba, ba};
-            cb[c4] = f32x4{bb, bb, bb, bb};
+            ca[c4] = ba;
+            cb[c4] = bb;
         }
         // W2 fragments stream through per k-block: eight independent MFMAs each
 #pragma unroll
@@ -496,36 +521,60 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
 #pragma unroll
             for (int c4 = 0; c4 < 4; ++c4) {
                 if constexpr (SC == SC_MLP_FP8) {
-                    ca[c4] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)af[c4][kb], (long)wa, ca[c4], 0, 0, 0);
-                    cb[c4] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)af[c4][kb], (long)wb, cb[c4], 0, 0, 0);
+                    ca[c4] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)wa, (long)af[c4][kb], ca[c4], 0, 0, 0);
+                    cb[c4] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)wb, (long)af[c4][kb], cb[c4], 0, 0, 0);
                 } else {
-                    ca[c4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c4][kb], wa, ca[c4], 0, 0, 0);
-                    cb[c4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[c4][kb], wb, cb[c4], 0, 0, 0);
+                    ca[c4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, af[c4][kb], ca[c4], 0, 0, 0);
+                    cb[c4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb, af[c4][kb], cb[c4], 0, 0, 0);
                 }
             }
         }
-        const h2 w3p = __builtin_bit_cast(h2, ml.w3h[16 * np + j]);
+    };
+    auto epilogue = [&](int np, const f32x4 (&ca)[4], const f32x4 (&cb)[4]) {
+        // w3 of this lane's hidden units: {4q, 4q+1}, {4q+2, 4q+3} of both n-tiles
+        const uint4 w3v = *reinterpret_cast<const uint4 *>(ml.w3h + (np * 4 + q) * 4);
+        // GELU of each candidate's eight hidden units as one 8-wide vector
 #pragma unroll
-        for (int c4 = 0; c4 < 4; ++c4)
+        for (int c4 = 0; c4 < 4; ++c4) {
+            float z[8];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float za = SC == SC_MLP_FP8 ? ca[c4][r] * w.descale : ca[c4][r];
-                const float zb = SC == SC_MLP_FP8 ? cb[c4][r] * w.descale : cb[c4][r];
-                const h2 x = pk_rtz(za, zb);
-                part[c4][r] = __builtin_amdgcn_fdot2(x * gelu_sig_h2(x, q2), w3p, part[c4][r], false);
+                z[r] = SC == SC_MLP_FP8 ? ca[c4][r] * w.descale : ca[c4][r];
+                z[4 + r] = SC == SC_MLP_FP8 ? cb[c4][r] * w.descale : cb[c4][r];
             }
-    }
+            h8 x;
 #pragma unroll
-    for (int c4 = 0; c4 < 4; ++c4)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            float v = part[c4][r];
-            v += __shfl_xor(v, 1);
-            v += __shfl_xor(v, 2);
-            v += __shfl_xor(v, 4);
-            v += __shfl_xor(v, 8);
-            if (j == 0) ml.sc[c4 * 16 + 4 * q + r] = v + w.b3;
+            for (int p2 = 0; p2 < 4; ++p2) {
+                const h2 xp = pk_rtz(z[2 * p2], z[2 * p2 + 1]);
+                x[2 * p2] = xp[0];
+                x[2 * p2 + 1] = xp[1];
+            }
+            const h8 g = x * gelu_sig_v<h8>(x, q2v);
+            float acc3 = part[c4];
+            acc3 = __builtin_amdgcn_fdot2(h2{g[0], g[1]}, __builtin_bit_cast(h2, w3v.x), acc3, false);
+            acc3 = __builtin_amdgcn_fdot2(h2{g[2], g[3]}, __builtin_bit_cast(h2, w3v.y), acc3, false);
+            acc3 = __builtin_amdgcn_fdot2(h2{g[4], g[5]}, __builtin_bit_cast(h2, w3v.z), acc3, false);
+            part[c4] = __builtin_amdgcn_fdot2(h2{g[6], g[7]}, __builtin_bit_cast(h2, w3v.w), acc3, false);
         }
+    };
+    f32x4 xa[4], xb[4], ya[4], yb[4];
+    mfma_pair(0, xa, xb);
+    for (int np = 0; np < 6; np += 2) {
+        mfma_pair(np + 1, ya, yb);
+        epilogue(np, xa, xb);
+        mfma_pair(np + 2, xa, xb);
+        epilogue(np + 1, ya, yb);
+    }
+    mfma_pair(7, ya, yb);
+    epilogue(6, xa, xb);
+    epilogue(7, ya, yb);
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) {
+        float v = part[c4];
+        v += __shfl_xor(v, 16);
+        v += __shfl_xor(v, 32);
+        if (q == 0) ml.sc[c4 * 16 + j] = v + w.b3;
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -753,8 +802,10 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             vec[MLP_H + i] = a.mlp.b2[i] * zs;
         }
         for (uint32_t i = threadIdx.x; i < MLP_H / 2; i += blockDim.x) {
-            const uint32_t p = i >> 4, jj = i & 15;
-            const h2 pr{(_Float16)a.mlp.w3[32 * p + jj], (_Float16)a.mlp.w3[32 * p + 16 + jj]};
+            // [np][q][k]: the pairs {4q, 4q+1}, {4q+2, 4q+3} of n-tile 2np (k = 0, 1) and 2np+1 (k = 2, 3)
+            const uint32_t p = i >> 4, qq = (i >> 2) & 3u, k = i & 3u;
+            const uint32_t h = 32 * p + (k >= 2 ? 16 : 0) + 4 * qq + 2 * (k & 1u);
+            const h2 pr{(_Float16)a.mlp.w3[h], (_Float16)a.mlp.w3[h + 1]};
             w3h[i] = __builtin_bit_cast(uint32_t, pr);
         }
         char *wv = base + w2b + MLP_VEC_BYTES + (size_t)wave * MLP_WAVE_BYTES;
