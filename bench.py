@@ -46,10 +46,11 @@ VALU_LANE_INSTR = 2222
 VALU_PEAK = 256 * 128 * 2.4e9  # lane-instructions/s: 256 CUs x 128 lanes/cycle x 2.4 GHz
 SCORE_TRAFFIC_PER_LAUNCH = 175.1e6 / 11
 # Multiply-class and 64-bit-add instructions among the 2222 of one candidate:
-# counted in the ISA of the compiled k_score (quadratic scorer, fast
-# mutation), in the two straight-line blocks of a normal candidate (parent
-# picks, crossover bits and gene loads; mutation draws, mutation and scoring).
-VALU_CLASS_PER_CANDIDATE = {"v_mad_u64_u32": 97, "v_mul_lo_u32": 223, "v_mul_hi_u32": 36, "v_lshl_add_u64": 91}
+# counted in the ISA of the production k_score (quadratic scorer, elites in
+# LDS, fast mutation), in the two straight-line blocks of a normal candidate
+# (parent picks, crossover bits and gene reads; mutation draws, mutation and
+# scoring), 1,909 VALU instructions together.
+VALU_CLASS_PER_CANDIDATE = {"v_mad_u64_u32": 69, "v_mul_lo_u32": 251, "v_mul_hi_u32": 36, "v_lshl_add_u64": 62}
 
 
 def valu_model(dev_rates, cand_per_s):

@@ -194,18 +194,21 @@ VALU_CLASSES = ("v_add_u32", "v_mul_lo_u32", "v_mul_hi_u32", "v_mad_u64_u32", "v
 
 def valu_rates(device: int = 0, iters: int = 1 << 15) -> dict:
     """Measured issue cost of each instruction class at full occupancy: ns
-    and shader-clock cycles per instruction per SIMD (the last class: per
-    MFMA, with four independent v_pk_fma_f16 issued beside each), the clock
-    those imply, and the SIMD count."""
+    and s_memtime counts per instruction per SIMD (the last class: per MFMA,
+    with four independent v_pk_fma_f16 issued beside each), the s_memtime
+    rate those imply, and the SIMD count.  One untimed pass of the MFMA class
+    first brings the device out of idle clocks."""
     ns_out, cyc_out = {}, {}
     simds = C.c_uint32(0)
+    warm = C.c_double(0.0)
+    check(lib().cs_valu_rate_probe(device, 8, iters, C.byref(warm), None, None))
     for op, name in enumerate(VALU_CLASSES):
         ns, cyc = C.c_double(0.0), C.c_double(0.0)
         check(lib().cs_valu_rate_probe(device, op, iters, C.byref(ns), C.byref(cyc), C.byref(simds)))
         ns_out[name] = ns.value / 8.0
         cyc_out[name] = cyc.value / 8.0
     ghz = sum(cyc_out.values()) / sum(ns_out.values())
-    return {"ns_per_instr": ns_out, "cycles_per_instr": cyc_out, "clock_ghz": ghz, "simds": simds.value}
+    return {"ns_per_instr": ns_out, "memtime_per_instr": cyc_out, "memtime_ghz": ghz, "simds": simds.value}
 
 
 def format_gene(v: float) -> str:

@@ -10,8 +10,8 @@
 // one class (inline asm, so the compiler neither folds nor reorders them), on
 // a grid of 8 waves per SIMD; two classes time the matrix core alone and
 // with packed-f16 work beside it.  No memory is touched in the loop; one
-// store per thread keeps the results live, and each wave's shader-clock
-// count (s_memtime) gives the cost in cycles independent of the clock.
+// store per thread keeps the results live, and each wave's s_memtime count
+// gives the cost in that counter's ticks beside the event-timed ns.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 

@@ -11,11 +11,11 @@ def test_valu_rate_probe_measures_every_class():
     from csfunnel import screen as scr
     r = scr.valu_rates(0, iters=1 << 12)
     ns = r["ns_per_instr"]
-    print({k: round(v, 4) for k, v in ns.items()}, {k: round(v, 3) for k, v in r["cycles_per_instr"].items()},
-          r["clock_ghz"], r["simds"], flush=True)
+    print({k: round(v, 4) for k, v in ns.items()}, {k: round(v, 3) for k, v in r["memtime_per_instr"].items()},
+          r["memtime_ghz"], r["simds"], flush=True)
     assert r["simds"] >= 4 and set(ns) == set(scr.VALU_CLASSES)
     assert all(math.isfinite(v) and v > 0 for v in ns.values())
-    assert 0.5 < r["clock_ghz"] < 5.0
+    assert 0.01 < r["memtime_ghz"] < 10.0
     assert ns["v_add_u32"] <= ns["v_mul_lo_u32"] * 1.05
 
 
