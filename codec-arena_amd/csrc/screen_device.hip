@@ -45,6 +45,15 @@ constexpr uint64_t MIX2 = 0x94D049BB133111EBull;
 
 constexpr int BLOCK_LOG = 16;
 constexpr uint64_t BLOCK = 1ull << BLOCK_LOG;  // draws per block
+// A scoring run covers up to two blocks.  Queue codes of its candidates:
+// start - run start (< 2 BLOCK + 62: 18 bits), index - first index (at most
+// 2 BLOCK / 33 < 4096 candidates: 12 bits), immigrant flag (bit 30), jump
+// flag (bit 31).
+constexpr int RUN_POS_BITS = 18;
+constexpr int RUN_IDX_BITS = 12;
+static_assert(2 * BLOCK + 62 <= (1ull << RUN_POS_BITS) && 2 * BLOCK / 33 < (1ull << RUN_IDX_BITS) &&
+                  RUN_POS_BITS + RUN_IDX_BITS == 30,
+              "queue code layout");
 constexpr int SUBSEG = 4096;                   // draws per walked sub-segment
 constexpr int NSUB = (int)(BLOCK / SUBSEG);    // 16
 constexpr int BITWORDS = (int)(BLOCK / 64) + 1;  // + 64 draws of look-ahead
@@ -660,9 +669,9 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
     const bool imm = IMM == 2 ? imm_rt : IMM == 1;
     const bool act = (uint32_t)lane < count;
     const uint32_t code = act ? q[(head + (uint32_t)lane) & (QCAP - 1)] : 0u;
-    const uint64_t c = bstart + (code & 0xffffu);
-    const uint64_t cidx = first + ((code >> 16) & 0x7ffu);
-    const bool jp = (code >> 28) & 1u;
+    const uint64_t c = bstart + (code & ((1u << RUN_POS_BITS) - 1u));
+    const uint64_t cidx = first + ((code >> RUN_POS_BITS) & ((1u << RUN_IDX_BITS) - 1u));
+    const bool jp = (code >> 31) & 1u;
     uint64_t s = seed + (c + 1) * GAMMA;
     const uint32_t ia = fastmod(draw_hi(s), a.ne_magic, a.ne);
     s += GAMMA;
@@ -821,15 +830,21 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
     uint32_t *qN = qbase + wave * 2 * QCAP;
     uint32_t *qI = qN + QCAP;
     unsigned long long screened = 0;
-    for (uint64_t fb = (uint64_t)blockIdx.x * NW + wave; fb < a.nflat; fb += (uint64_t)gridDim.x * NW) {
+    // Each wave takes pairs of flat blocks; two blocks of one stream are
+    // walked as one run (the chain crosses the block boundary by itself), so
+    // the partly filled batches at the end of a run come once per two blocks.
+    for (uint64_t fp = (uint64_t)blockIdx.x * NW + wave; 2 * fp < a.nflat; fp += (uint64_t)gridDim.x * NW) {
+    const uint64_t fend = min(2 * fp + 2, a.nflat);
+    for (uint64_t fb = 2 * fp, nb = 1; fb < fend; fb += nb) {
         const int t = find_seg(a.seg_first, a.T, fb);
+        nb = (fb + 1 < fend && fb + 1 < a.seg_first[t + 1]) ? 2 : 1;
         const LaunchSeg sg = a.segs[t];
         const StreamInfo st = a.streams[t];
         const uint64_t b = sg.b0 + (fb - sg.flat_first);
         const uint64_t ent = a.entries[st.blk_first + b];
         const uint64_t first = ent >> 8;
         if (first >= st.my_lambda) continue;
-        const uint64_t bstart = b << BLOCK_LOG, bend = bstart + BLOCK;
+        const uint64_t bstart = b << BLOCK_LOG, bend = bstart + nb * BLOCK;
         uint64_t c = bstart + (ent & 255u);
         uint64_t idx = first;
         uint32_t headN = 0, tailN = 0, headI = 0, tailI = 0;
@@ -848,8 +863,8 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             const uint64_t lam = st.my_lambda - idx;
             n = min(n, min(room, lam));
             const bool take = (uint64_t)lane < n;
-            const uint32_t code = (uint32_t)(cj - bstart) | ((uint32_t)(idx - first + (uint64_t)lane) << 16) |
-                                  ((uint32_t)im << 27) | ((uint32_t)jp << 28);
+            const uint32_t code = (uint32_t)(cj - bstart) | ((uint32_t)(idx - first + (uint64_t)lane) << RUN_POS_BITS) |
+                                  ((uint32_t)im << 30) | ((uint32_t)jp << 31);
             const uint64_t mN = __ballot(take && !im), mI = __ballot(take && im);
             if (take && !im) qN[(tailN + (uint32_t)__popcll(mN & lt)) & (QCAP - 1)] = code;
             if (take && im) qI[(tailI + (uint32_t)__popcll(mI & lt)) & (QCAP - 1)] = code;
@@ -917,6 +932,7 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             }
         }
         __builtin_amdgcn_wave_barrier();
+    }
     }
     if (lane == 0 && screened) atomicAdd(a.cand_count, screened);
 }
