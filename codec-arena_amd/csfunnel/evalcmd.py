@@ -228,22 +228,28 @@ def eval_cmd(argv: List[str], threads: Optional[int] = None) -> int:
 
         nthreads = threads or int(os.environ.get("CSFUNNEL_EVAL_THREADS", 0)) or min(16, os.cpu_count() or 1)
         n_ok = 0
+        # genomes go to the pool a chunk at a time, so the held results (and
+        # decoded clips, with --dump-recon) stay bounded
+        chunk = 1 if dump else 64
+        nc = len(clips)
         with open(out_path, "w") as out, ThreadPoolExecutor(nthreads) as ex:
-            res = list(ex.map(job, range(len(genomes) * len(clips))))
-            for gi, g in enumerate(genomes):
-                row = res[gi * len(clips):(gi + 1) * len(clips)]
-                bad = next((k for k, r in enumerate(row) if r[0] is None), None)
-                if dump:
-                    os.makedirs(dump, exist_ok=True)
-                    for k, r in enumerate(row[:len(row) if bad is None else bad]):
-                        with open(os.path.join(dump, clips[k][0] + ".recon.yuv"), "wb") as f:
-                            f.write(r[2])
-                if bad is not None:
-                    sys.stderr.write(f"[evalClip error: error.{row[bad][1]} clip={clips[bad][0]}]\n")
-                    n_bad += 1
-                    continue
-                out.write(_genome_line(g, [r[0] for r in row], clips))
-                n_ok += 1
+            for g0 in range(0, len(genomes), chunk):
+                g1 = min(len(genomes), g0 + chunk)
+                res = list(ex.map(job, range(g0 * nc, g1 * nc)))
+                for gi in range(g0, g1):
+                    row = res[(gi - g0) * nc:(gi - g0 + 1) * nc]
+                    bad = next((k for k, r in enumerate(row) if r[0] is None), None)
+                    if dump:
+                        os.makedirs(dump, exist_ok=True)
+                        for k, r in enumerate(row[:nc if bad is None else bad]):
+                            with open(os.path.join(dump, clips[k][0] + ".recon.yuv"), "wb") as f:
+                                f.write(r[2])
+                    if bad is not None:
+                        sys.stderr.write(f"[evalClip error: error.{row[bad][1]} clip={clips[bad][0]}]\n")
+                        n_bad += 1
+                        continue
+                    out.write(_genome_line(genomes[gi], [r[0] for r in row], clips))
+                    n_ok += 1
         sys.stderr.write(f"eval: {n_ok} ok, {n_bad} bad, {len(clips)} clips\n")
         return 0
     except CmdError as e:
