@@ -334,6 +334,7 @@ def main():
                                     "probe": valu_model(rates, LAMBDA * args.steps / (ms_score * 1e-3))},
                      "flop_per_candidate": FLOP_PER_CANDIDATE, "launches": launches,
                      "avg_launch_ms": ms_score / max(1, launches), "k_map_ms_per_step": ms_map / args.steps,
+                     "walk_fixes": sum(s.get("walk_fixes", 0) for s in stats),
                      "k_score_ms_per_step": ms_score / args.steps},
         "published_cpu_reference": {"value": PUBLISHED_CPU, "unit": "candidates/s", "cores": 4,
                                     "source": "docs/SCALE.md:26 (Zig SIMD, 4 threads)"},

@@ -301,7 +301,7 @@ __global__ __launch_bounds__(64) void k_down(const uint64_t *__restrict__ in,
 constexpr int SEG_LOG = 25;
 constexpr uint64_t SEG = 1ull << SEG_LOG;
 constexpr int SEG_BLOCKS_LOG = SEG_LOG - BLOCK_LOG;  // 512 blocks per segment
-constexpr uint64_t BURN = 1ull << 23;
+constexpr uint64_t BURN = 1ull << 22;  // ~14x the mean merge distance
 
 struct SegOut {
     uint64_t x;      // first candidate start >= segment start
