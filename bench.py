@@ -67,14 +67,16 @@ def valu_model(dev_rates, cand_per_s):
             "cost_vs_add": {c: round(ns[c] / ns["v_add_u32"], 3) for c in VALU_CLASS_PER_CANDIDATE}}
 
 
-def mfma_model(dev_rates, flop_per_s):
-    """The MLP screen's MFMA rate against the probe's v_mfma_f32_16x16x32_bf16
-    stream, and whether packed-f16 work issued beside the MFMAs is hidden
-    (cost of MFMA + 4 v_pk_fma_f16 against MFMA alone and their sum)."""
+def mfma_model(dev_rates, flop_per_s, scorer=1):
+    """The MLP screen's MFMA rate against the probe's stream of its layer-2
+    instruction (v_mfma_f32_16x16x32_bf16, or v_mfma_f32_16x16x128_f8f6f4 for
+    FP8), and whether packed-f16 work issued beside the MFMAs is hidden (cost
+    of MFMA + 4 v_pk_fma_f16 against MFMA alone and their sum)."""
     ns = dev_rates["ns_per_instr"]
-    probe = dev_rates["simds"] * 16384 / (ns["v_mfma_f32_16x16x32_bf16"] * 1e-9) / 1e12
-    return {"probe_mfma_tflops": probe, "frac_of_probe": flop_per_s / 1e12 / probe,
-            "ns_mfma": ns["v_mfma_f32_16x16x32_bf16"], "ns_mfma_plus_4pk": ns["mfma+4pk_fma_f16"],
+    ins, flop = (("v_mfma_f32_16x16x128_f8f6f4", 65536) if scorer == 2 else ("v_mfma_f32_16x16x32_bf16", 16384))
+    probe = dev_rates["simds"] * flop / (ns[ins] * 1e-9) / 1e12
+    return {"probe_instruction": ins, "probe_mfma_tflops": probe, "frac_of_probe": flop_per_s / 1e12 / probe,
+            "ns_mfma_bf16": ns["v_mfma_f32_16x16x32_bf16"], "ns_mfma_bf16_plus_4pk": ns["mfma+4pk_fma_f16"],
             "ns_4pk": 4 * ns["v_pk_fma_f16"]}
 
 
@@ -203,7 +205,7 @@ def mlp_line(scorer, steps, dt, stats, train_info, rates=None):
                          "unit": "TFLOP/s", "frac": achieved / PEAK_MFMA[scorer],
                          "flop_per_candidate": MLP_FLOP_PER_CANDIDATE, "launches": launches,
                          "avg_launch_ms": ms_score / max(1, launches), "k_score_ms_per_step": ms_score / steps,
-                         **({"probe": mfma_model(rates, achieved * 1e12)} if rates and scorer == 1 else {})},
+                         **({"probe": mfma_model(rates, achieved * 1e12, scorer)} if rates else {})},
             "surrogate": {"train_pairs": 20000, "epochs": 300, "train_s": round(train_info["train_s"], 2),
                           "final_loss": float(train_info["final_loss"]),
                           "labels": "PSNR - 14 bpp from the codec label source (ten-clip corpus)",
@@ -289,11 +291,18 @@ def main():
     funnel = None
     if not args.no_mlp:
         mlp, wq, train_info = train_surrogate(dev)
-        train_info.pop("genes")
+        genes = train_info.pop("genes")
         mdt, mstats = run_generations(dev, model, elites, 1, mlp, 2, 1, 64)
         raw, rr = mlp_recall(dev, elites, 1, mlp)
         recall.update({"mlp_bf16": raw, "mlp_bf16_after_rerank": rr, "k_mlp": 64})
         mlp_out = mlp_line(1, 2, mdt, mstats, train_info, rates)
+        # the same network with layer 2 in FP8 (per-tensor power-of-two scales)
+        mlp.calibrate_fp8(genes)
+        fdt, fstats = run_generations(dev, model, elites, 2, mlp, 2, 1, 64)
+        fraw, frr = mlp_recall(dev, elites, 2, mlp)
+        recall.update({"mlp_fp8": fraw, "mlp_fp8_after_rerank": frr})
+        fp8_line = mlp_line(2, 2, fdt, fstats, train_info, rates)
+        mlp_out["fp8"] = {k: fp8_line[k] for k in ("value", "unit", "ms_per_step", "dtype", "roofline")}
         if args.funnel_gens > 0:
             from csfunnel import codec, surrogate
             print("funnel: quadratic vs MLP screen, survivors re-scored by the codec label source", file=sys.stderr,
@@ -335,6 +344,8 @@ def main():
                      "flop_per_candidate": FLOP_PER_CANDIDATE, "launches": launches,
                      "avg_launch_ms": ms_score / max(1, launches), "k_map_ms_per_step": ms_map / args.steps,
                      "walk_fixes": sum(s.get("walk_fixes", 0) for s in stats),
+                     "host_replay_ms_per_step": sum(s["ms_replay"] for s in stats) / args.steps,
+                     "cs_screen_ms_per_step": sum(s["ms_total"] for s in stats) / args.steps,
                      "k_score_ms_per_step": ms_score / args.steps},
         "published_cpu_reference": {"value": PUBLISHED_CPU, "unit": "candidates/s", "cores": 4,
                                     "source": "docs/SCALE.md:26 (Zig SIMD, 4 threads)"},

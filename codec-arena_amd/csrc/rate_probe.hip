@@ -33,10 +33,12 @@ enum RateOp {
     RATE_PK_FMA_F16 = 7,
     RATE_MFMA_BF16 = 8,     // v_mfma_f32_16x16x32_bf16
     RATE_MFMA_PK_F16 = 9,   // the same MFMAs with four independent v_pk_fma_f16 after each
-    RATE_NOPS = 10
+    RATE_MFMA_F8 = 10,      // v_mfma_f32_16x16x128_f8f6f4 (E4M3 x E4M3)
+    RATE_NOPS = 11
 };
 typedef __bf16 pbf16x8 __attribute__((ext_vector_type(8)));
 typedef float pf32x4 __attribute__((ext_vector_type(4)));
+typedef int pi32x8 __attribute__((ext_vector_type(8)));
 
 template <int OP>
 __global__ __launch_bounds__(256) void k_rate(uint32_t *out, uint64_t *cyc, uint32_t iters) {
@@ -47,11 +49,14 @@ __global__ __launch_bounds__(256) void k_rate(uint32_t *out, uint64_t *cyc, uint
     float f[8];
     pf32x4 acc[8];
     pbf16x8 ma, mb;
+    pi32x8 fa, fb;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         ma[i] = (__bf16)(1.0f + (float)(tid & 7) * 0.125f);
         mb[i] = (__bf16)(0.5f - (float)i * 0.0625f);
         acc[i] = pf32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        fa[i] = 0x38383838 ^ (int)(tid & 7);  // E4M3 values near 0.5
+        fb[i] = 0x30303030 + i;
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -78,6 +83,8 @@ __global__ __launch_bounds__(256) void k_rate(uint32_t *out, uint64_t *cyc, uint
             if constexpr (OP == RATE_PK_FMA_F16) asm volatile("v_pk_fma_f16 %0, %0, %1, %1" : "+v"(a[i]) : "v"(c));
             if constexpr (OP == RATE_MFMA_BF16 || OP == RATE_MFMA_PK_F16)
                 acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ma, mb, acc[i], 0, 0, 0);
+            if constexpr (OP == RATE_MFMA_F8)
+                acc[i] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(fa, fb, acc[i], 0, 0, 0, 0, 0, 0);
             if constexpr (OP == RATE_MFMA_PK_F16) {
                 asm volatile("v_pk_fma_f16 %0, %0, %1, %1" : "+v"(a[i]) : "v"(c));
                 asm volatile("v_pk_fma_f16 %0, %0, %1, %1" : "+v"(a[(i + 1) & 7]) : "v"(c));
@@ -120,8 +127,8 @@ extern "C" int cs_valu_rate_probe(int device, int op, uint32_t iters, double *ns
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     typedef void (*Kern)(uint32_t *, uint64_t *, uint32_t);
-    static const Kern kerns[RATE_NOPS] = {k_rate<0>, k_rate<1>, k_rate<2>, k_rate<3>, k_rate<4>,
-                                          k_rate<5>, k_rate<6>, k_rate<7>, k_rate<8>, k_rate<9>};
+    static const Kern kerns[RATE_NOPS] = {k_rate<0>, k_rate<1>, k_rate<2>, k_rate<3>, k_rate<4>, k_rate<5>,
+                                          k_rate<6>, k_rate<7>, k_rate<8>, k_rate<9>, k_rate<10>};
     int rc = CS_OK;
     float ms = 0.0f;
     // one warm-up launch, then the timed one

@@ -522,6 +522,25 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
             ca[c4] = ba;
             cb[c4] = bb;
         }
+        if constexpr (SC == SC_MLP_FP8) {
+            // CDNA4's v_mfma_f32_16x16x128_f8f6f4 (E4M3 operands, no block
+            // scales): four k-blocks per instruction, W2 staged as 32-byte
+            // fragments [k-quad][n-tile][lane] and H1 as the four k-blocks'
+            // 8-byte fragments side by side, in the same k order.
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const i32x8 wa = reinterpret_cast<const i32x8 *>(ml.w2)[(m * 16 + 2 * np) * 64 + lane];
+                const i32x8 wb = reinterpret_cast<const i32x8 *>(ml.w2)[(m * 16 + 2 * np + 1) * 64 + lane];
+#pragma unroll
+                for (int c4 = 0; c4 < 4; ++c4) {
+                    const i32x8 hx = __builtin_bit_cast(
+                        i32x8, u64x4{(uint64_t)af[c4][4 * m], (uint64_t)af[c4][4 * m + 1], (uint64_t)af[c4][4 * m + 2],
+                                     (uint64_t)af[c4][4 * m + 3]});
+                    ca[c4] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(wa, hx, ca[c4], 0, 0, 0, 0, 0, 0);
+                    cb[c4] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(wb, hx, cb[c4], 0, 0, 0, 0, 0, 0);
+                }
+            }
+        } else {
         // W2 fragments stream through per k-block: eight independent MFMAs each
 #pragma unroll
         for (int kb = 0; kb < 8; ++kb) {
@@ -529,14 +548,10 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
             const Frag wb = reinterpret_cast<const Frag *>(ml.w2)[(kb * 16 + 2 * np + 1) * 64 + lane];
 #pragma unroll
             for (int c4 = 0; c4 < 4; ++c4) {
-                if constexpr (SC == SC_MLP_FP8) {
-                    ca[c4] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)wa, (long)af[c4][kb], ca[c4], 0, 0, 0);
-                    cb[c4] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)wb, (long)af[c4][kb], cb[c4], 0, 0, 0);
-                } else {
-                    ca[c4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, af[c4][kb], ca[c4], 0, 0, 0);
-                    cb[c4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb, af[c4][kb], cb[c4], 0, 0, 0);
-                }
+                ca[c4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, af[c4][kb], ca[c4], 0, 0, 0);
+                cb[c4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb, af[c4][kb], cb[c4], 0, 0, 0);
             }
+        }
         }
     };
     auto epilogue = [&](int np, const f32x4 (&ca)[4], const f32x4 (&cb)[4]) {
@@ -800,8 +815,18 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
         const size_t w2b = SC == SC_MLP_FP8 ? MLP_W2_FP8_BYTES : MLP_W2_BF16_BYTES;
         const uint4 *src = SC == SC_MLP_FP8 ? reinterpret_cast<const uint4 *>(a.mlp.w2f8)
                                             : reinterpret_cast<const uint4 *>(a.mlp.w2f);
-        uint4 *dst = reinterpret_cast<uint4 *>(base);
-        for (uint32_t i = threadIdx.x; i < w2b / 16; i += blockDim.x) dst[i] = src[i];
+        if constexpr (SC == SC_MLP_FP8) {
+            // [k-block][n-tile][lane] x 8 bytes -> [k-quad][n-tile][lane] x 32 bytes
+            const uint64_t *s8 = reinterpret_cast<const uint64_t *>(src);
+            uint64_t *d8 = reinterpret_cast<uint64_t *>(base);
+            for (uint32_t i = threadIdx.x; i < 8u * 16u * 64u; i += blockDim.x) {
+                const uint32_t kb = i >> 10, nt = (i >> 6) & 15u, ln = i & 63u;
+                d8[(((kb >> 2) * 16u + nt) * 64u + ln) * 4u + (kb & 3u)] = s8[i];
+            }
+        } else {
+            uint4 *dst = reinterpret_cast<uint4 *>(base);
+            for (uint32_t i = threadIdx.x; i < w2b / 16; i += blockDim.x) dst[i] = src[i];
+        }
         const float hs = SC == SC_MLP_FP8 ? a.mlp.h_scale : 1.0f;
         const float zs = SC == SC_MLP_FP8 ? a.mlp.h_scale * a.mlp.w_scale : 1.0f;
         float *vec = reinterpret_cast<float *>(base + w2b);
@@ -1079,11 +1104,16 @@ static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std
     if (cnt) {
         if (ctx->host_rec.size() < cnt) ctx->host_rec.resize(cnt);
         HIPCHK(hipMemcpy(ctx->host_rec.data(), a.rec, sizeof(Record) * cnt, hipMemcpyDeviceToHost));
-        std::sort(ctx->host_rec.begin(), ctx->host_rec.begin() + cnt, [](const Record &x, const Record &y) {
-            return x.stream != y.stream ? x.stream < y.stream : x.index < y.index;
+        // replay order (stream, candidate index): sort 16-byte keys, not the records
+        std::vector<std::pair<uint64_t, uint64_t>> order(cnt);
+        for (uint32_t i = 0; i < cnt; ++i)
+            order[i] = {((uint64_t)ctx->host_rec[i].stream << 32) | i, ctx->host_rec[i].index};
+        std::sort(order.begin(), order.end(), [](const std::pair<uint64_t, uint64_t> &x,
+                                                 const std::pair<uint64_t, uint64_t> &y) {
+            return (x.first >> 32) != (y.first >> 32) ? (x.first >> 32) < (y.first >> 32) : x.second < y.second;
         });
-        for (uint32_t i = 0; i < cnt; ++i) {
-            const Record &r = ctx->host_rec[i];
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const Record &r = ctx->host_rec[(uint32_t)order[k].first];
             cs_survivor s;
             s.score = r.score;
             std::memcpy(s.g, r.g, sizeof s.g);
