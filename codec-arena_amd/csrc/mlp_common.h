@@ -65,9 +65,8 @@ __device__ __forceinline__ h2 pk_rtz(float a, float b) { return __builtin_bit_ca
 // packed chains, so each Horner step issues four v_pk_fma_f16 back to back
 // instead of waiting on one dependent chain.
 template <typename V>
-__device__ __forceinline__ V gelu_sig_v(V x, V q) {
+__device__ __forceinline__ V gelu_sig_scaled(V v) {  // S from v = x q
     const V one = (V)(_Float16)1.0f;
-    const V v = x * q;
     const V w = __builtin_elementwise_min(__builtin_elementwise_fma(v, v, -one), one);
     V p = (V)(_Float16)0.022354126f;
     p = __builtin_elementwise_fma(p, w, (V)(_Float16)-0.0514831543f);
@@ -78,6 +77,10 @@ __device__ __forceinline__ V gelu_sig_v(V x, V q) {
     p = __builtin_elementwise_fma(p, w, (V)(_Float16)0.497802734f);
     const V sg = __builtin_elementwise_fma(v, p, (V)(_Float16)0.5f);
     return __builtin_elementwise_min(__builtin_elementwise_max(sg, (V)(_Float16)0.0f), one);
+}
+template <typename V>
+__device__ __forceinline__ V gelu_sig_v(V x, V q) {
+    return gelu_sig_scaled<V>(x * q);
 }
 __device__ __forceinline__ h2 gelu_sig_h2(h2 x, h2 q) { return gelu_sig_v<h2>(x, q); }
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));

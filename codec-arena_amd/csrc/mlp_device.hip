@@ -210,8 +210,9 @@ __global__ void k_refresh(const float *__restrict__ p, __bf16 *__restrict__ w1b,
         const int mt = i / 64, lane = i % 64, h = 16 * mt + (lane & 15), k0 = 4 * (lane >> 4);
         bf16x4 v, vh;
         for (int e = 0; e < 4; ++e) {
-            v[e] = (__bf16)p[P_W1 + (k0 + e) * MLP_H + h];
-            vh[e] = (__bf16)((float)v[e] * h_scale);  // exact: h_scale is a power of two
+            const float w1 = p[P_W1 + (k0 + e) * MLP_H + h];
+            v[e] = (__bf16)(w1 * GELU_Q);  // the BF16 screen's layer-1 GELU input scale, folded in
+            vh[e] = (__bf16)((float)(__bf16)w1 * h_scale);  // exact: h_scale is a power of two
         }
         w1f[i] = v;
         w1fh[i] = vh;

@@ -481,7 +481,14 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
                 xb[2 * t] = x23[0];
                 xb[2 * t + 1] = x23[1];
             }
-            const h8 ga = xa * gelu_sig_v<h8>(xa, q1v), gb = xb * gelu_sig_v<h8>(xb, q1v);
+            h8 ga, gb;
+            if constexpr (SC == SC_MLP_FP8) {
+                ga = xa * gelu_sig_v<h8>(xa, q1v);
+                gb = xb * gelu_sig_v<h8>(xb, q1v);
+            } else {  // xa, xb are v = x q; H1 carries the factor q on to layer 2
+                ga = xa * gelu_sig_scaled<h8>(xa);
+                gb = xb * gelu_sig_scaled<h8>(xb);
+            }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
             const int mt = m4 + t;
@@ -573,7 +580,7 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
                 x[2 * p2] = xp[0];
                 x[2 * p2 + 1] = xp[1];
             }
-            const h8 g = x * gelu_sig_v<h8>(x, q2v);
+            const h8 g = SC == SC_MLP_FP8 ? x * gelu_sig_v<h8>(x, q2v) : x * gelu_sig_scaled<h8>(x);
             float acc3 = part[c4];
             acc3 = __builtin_amdgcn_fdot2(h2{g[0], g[1]}, __builtin_bit_cast(h2, w3v.x), acc3, false);
             acc3 = __builtin_amdgcn_fdot2(h2{g[2], g[3]}, __builtin_bit_cast(h2, w3v.y), acc3, false);
@@ -827,8 +834,10 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             uint4 *dst = reinterpret_cast<uint4 *>(base);
             for (uint32_t i = threadIdx.x; i < w2b / 16; i += blockDim.x) dst[i] = src[i];
         }
-        const float hs = SC == SC_MLP_FP8 ? a.mlp.h_scale : 1.0f;
-        const float zs = SC == SC_MLP_FP8 ? a.mlp.h_scale * a.mlp.w_scale : 1.0f;
+        // BF16: the GELU input scale q is folded into b1, b2 (and W1, 1/q into
+        // w3), so the accumulators hold v = x q directly
+        const float hs = SC == SC_MLP_FP8 ? a.mlp.h_scale : GELU_Q;
+        const float zs = SC == SC_MLP_FP8 ? a.mlp.h_scale * a.mlp.w_scale : GELU_Q;
         float *vec = reinterpret_cast<float *>(base + w2b);
         uint32_t *w3h = reinterpret_cast<uint32_t *>(vec + 2 * MLP_H);
         for (uint32_t i = threadIdx.x; i < MLP_H; i += blockDim.x) {
@@ -839,7 +848,8 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             // [np][q][k]: the pairs {4q, 4q+1}, {4q+2, 4q+3} of n-tile 2np (k = 0, 1) and 2np+1 (k = 2, 3)
             const uint32_t p = i >> 4, qq = (i >> 2) & 3u, k = i & 3u;
             const uint32_t h = 32 * p + (k >= 2 ? 16 : 0) + 4 * qq + 2 * (k & 1u);
-            const h2 pr{(_Float16)a.mlp.w3[h], (_Float16)a.mlp.w3[h + 1]};
+            const float w3s = SC == SC_MLP_FP8 ? 1.0f : 1.0f / GELU_Q;
+            const h2 pr{(_Float16)(a.mlp.w3[h] * w3s), (_Float16)(a.mlp.w3[h + 1] * w3s)};
             w3h[i] = __builtin_bit_cast(uint32_t, pr);
         }
         char *wv = base + w2b + MLP_VEC_BYTES + (size_t)wave * MLP_WAVE_BYTES;

@@ -17,7 +17,7 @@ import gelu_poly_fit as fit  # noqa: E402
 
 def header_coefficients():
     txt = open(os.path.join(CSRC, "mlp_common.h")).read()
-    body = txt[txt.index("V gelu_sig_v(V x, V q)"):]
+    body = txt[txt.index("V gelu_sig_scaled(V v)"):]
     body = body[:body.index("return")]
     vals = [float(v) for v in re.findall(r"\(V\)\(_Float16\)(-?[0-9.]+)f", body)]
     # one, the leading coefficient, the Horner constants, 1/2, 0
