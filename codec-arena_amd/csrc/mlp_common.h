@@ -105,7 +105,7 @@ struct MlpScreenW {
     float b3;
     float h_scale;   // fp8: power of two; W1, b1 (so H1) are multiplied by it
     float w_scale;   // fp8: power of two; W2 was multiplied by it before conversion
-    float descale;   // 1 / (h_scale * w_scale)
+    float descale;   // q / (h_scale * w_scale), q = GELU_Q: FP8 layer-2 accumulators to v = q z2
     float q1;        // v = x q1 in the layer-1 GELU: GELU_Q / h_scale (fp8) or GELU_Q
 };
 

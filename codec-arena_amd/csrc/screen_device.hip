@@ -456,7 +456,7 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
     bf16x4 w1v[16];
 #pragma unroll
     for (int mt = 0; mt < 16; ++mt) w1v[mt] = w1f[mt * 64 + lane];
-    const h8 q1v = (h8)(_Float16)w.q1, q2v = (h8)(_Float16)GELU_Q;
+    const h8 q1v = (h8)(_Float16)w.q1;
     Frag af[4][8];
 #pragma unroll
     for (int c4 = 0; c4 < 4; ++c4) {
@@ -580,7 +580,7 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
                 x[2 * p2] = xp[0];
                 x[2 * p2 + 1] = xp[1];
             }
-            const h8 g = SC == SC_MLP_FP8 ? x * gelu_sig_v<h8>(x, q2v) : x * gelu_sig_scaled<h8>(x);
+            const h8 g = x * gelu_sig_scaled<h8>(x);  // x = q z2 (scale folded into W, b2 or the descale)
             float acc3 = part[c4];
             acc3 = __builtin_amdgcn_fdot2(h2{g[0], g[1]}, __builtin_bit_cast(h2, w3v.x), acc3, false);
             acc3 = __builtin_amdgcn_fdot2(h2{g[2], g[3]}, __builtin_bit_cast(h2, w3v.y), acc3, false);
@@ -834,8 +834,9 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             uint4 *dst = reinterpret_cast<uint4 *>(base);
             for (uint32_t i = threadIdx.x; i < w2b / 16; i += blockDim.x) dst[i] = src[i];
         }
-        // BF16: the GELU input scale q is folded into b1, b2 (and W1, 1/q into
-        // w3), so the accumulators hold v = x q directly
+        // The GELU input scale q: BF16 folds it into W1, b1, b2 and both
+        // screens 1/q into w3 (FP8 layer 2 applies q with the descale), so
+        // the accumulators hold v = x q directly
         const float hs = SC == SC_MLP_FP8 ? a.mlp.h_scale : GELU_Q;
         const float zs = SC == SC_MLP_FP8 ? a.mlp.h_scale * a.mlp.w_scale : GELU_Q;
         float *vec = reinterpret_cast<float *>(base + w2b);
@@ -848,7 +849,7 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             // [np][q][k]: the pairs {4q, 4q+1}, {4q+2, 4q+3} of n-tile 2np (k = 0, 1) and 2np+1 (k = 2, 3)
             const uint32_t p = i >> 4, qq = (i >> 2) & 3u, k = i & 3u;
             const uint32_t h = 32 * p + (k >= 2 ? 16 : 0) + 4 * qq + 2 * (k & 1u);
-            const float w3s = SC == SC_MLP_FP8 ? 1.0f : 1.0f / GELU_Q;
+            const float w3s = 1.0f / GELU_Q;  // both screens feed layer 3 with q GELU(z2)
             const h2 pr{(_Float16)(a.mlp.w3[h] * w3s), (_Float16)(a.mlp.w3[h + 1] * w3s)};
             w3h[i] = __builtin_bit_cast(uint32_t, pr);
         }
@@ -1404,7 +1405,8 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         mw.b3 = b3;
         mw.h_scale = ml->h_scale;
         mw.w_scale = ml->w_scale;
-        mw.descale = 1.0f / (ml->h_scale * ml->w_scale);
+        // FP8 layer 2: the descale also applies the GELU input scale q
+        mw.descale = GELU_Q / (ml->h_scale * ml->w_scale);
         mw.q1 = sc == SC_MLP_FP8 ? GELU_Q / ml->h_scale : GELU_Q;
     }
     ScoreArgs &a = h.sa;
