@@ -42,10 +42,10 @@ PEAK_FP32_TFLOPS = 157.3
 PUBLISHED_CPU = 17.2e6    # reference Zig SIMD screen, 4 threads (docs/SCALE.md:26)
 # measured once with rocprofv3 (profiles/screen_walk_offsets_2p65e9.md):
 # k_score VALU lane-instructions per candidate and HBM bytes per launch
-VALU_LANE_INSTR = 2177
+VALU_LANE_INSTR = 2139
 VALU_PEAK = 256 * 128 * 2.4e9  # lane-instructions/s: 256 CUs x 128 lanes/cycle x 2.4 GHz
 SCORE_TRAFFIC_PER_LAUNCH = 97.7e6 / 11
-# Multiply-class and 64-bit-add instructions among the 2177 of one candidate:
+# Multiply-class and 64-bit-add instructions among the 2139 of one candidate:
 # counted in the ISA of the production k_score (quadratic scorer, elites in
 # LDS, fast mutation), in the two straight-line blocks of a normal candidate
 # (parent picks, crossover bits and gene reads; mutation draws, mutation and

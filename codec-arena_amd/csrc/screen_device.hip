@@ -311,43 +311,58 @@ struct SegOut {
 
 // Follows one stream's chain from `c` (a candidate start) and records, for
 // every block boundary B in [seg_start, seg_end], the first start >= B and
-// the number of starts in [seg_start, that start).
+// the number of starts in [seg_start, that start).  The chain position c, B
+// and the counts are the same in every lane (scalar registers); lane j's
+// counter for draw c + 62 j + k is (seed + (c + k) gamma) + 62 j gamma, one
+// 64-bit add per lane.  A chain that passes seg_end without resolving it
+// (impossible: a round spans < 4032 draws) stores y = ~0 and stops, which the
+// host's chain check rejects.
 __device__ void walk_segment(uint64_t seed, uint64_t c, uint64_t seg_start, uint64_t seg_end, uint32_t immT,
                              uint32_t jumpT, uint64_t *__restrict__ entries_seg, SegOut *__restrict__ out,
                              int lane) {
+    uint64_t lane_step = (uint64_t)(62u * (uint32_t)lane) * GAMMA;
+    asm volatile("" : "+v"(lane_step));  // opaque: keeps (c + 3) gamma + lane_step from being refactored
     uint64_t B = seg_start;  // next block boundary to resolve
     uint64_t r = 0;          // starts >= seg_start seen so far
     for (;;) {
-        const uint64_t cj = c + 62ull * (uint64_t)lane;
-        const bool im = draw_u24(seed + (cj + 3) * GAMMA) < immT;
-        const bool jp = draw_u24(seed + (cj + (im ? 33u : 62u)) * GAMMA) < jumpT;
+        const uint64_t x3 = seed + (c + 3) * GAMMA + lane_step;  // draw c + 62 j + 3
+        const bool im = draw_u24(x3) < immT;
+        // the jump test: draw c + 62 j + 33 (immigrant) or + 62
+        const bool jp = draw_u24(x3 + (im ? 30ull * GAMMA : 59ull * GAMMA)) < jumpT;
         const uint32_t L = im ? (jp ? 35u : 33u) : (jp ? 64u : 62u);
-        const uint64_t stop = __ballot(L != 62u);
-        const uint64_t n = stop ? (uint64_t)__ffsll((unsigned long long)stop) : 64ull;
-        // valid starts this round: c + 62 j, j < n (ascending)
+        const uint64_t stop = __ballot(im || jp);
+        const uint32_t n = stop ? (uint32_t)__ffsll((unsigned long long)stop) : 64u;
+        // valid starts this round: c + 62 k, k < n (ascending); the last one
+        const uint64_t last = c + 62ull * (n - 1);
         // lanes before the segment start do not count
-        const uint64_t ks = c >= seg_start ? 0 : min(n, (seg_start - c + 61) / 62);
-        // a round spans < 4032 draws, so at most one boundary falls inside it
-        if (B <= seg_end) {
-            const uint64_t kb = c >= B ? 0 : (B - c + 61) / 62;
-            if (kb < n) {
-                const uint64_t first = c + 62ull * kb;
-                const uint64_t rel = r + (kb - min(kb, ks));
-                if (B == seg_end) {
-                    if (lane == 0) {
-                        out->y = first;
-                        out->total = rel;
-                    }
-                    return;
+        uint32_t ks = 0;
+        if (c < seg_start) ks = last < seg_start ? n : (uint32_t)(seg_start - c + 61) / 62u;
+        // a round spans < 4032 draws, so at most one boundary falls inside
+        // it; B <= last <=> the first start >= B is one of this round's
+        if (B <= last) {
+            const uint32_t kb = c >= B ? 0u : (uint32_t)(B - c + 61) / 62u;
+            const uint64_t first = c + 62ull * kb;
+            const uint64_t rel = r + (kb - min(kb, ks));
+            if (B == seg_end) {
+                if (lane == 0) {
+                    out->y = first;
+                    out->total = rel;
                 }
-                if (B == seg_start && lane == 0) out->x = first;
-                if (lane == 0) entries_seg[(B - seg_start) >> BLOCK_LOG] = (rel << 8) | (first - B);
-                B += BLOCK;
+                return;
             }
+            if (B == seg_start && lane == 0) out->x = first;
+            if (lane == 0) entries_seg[(B - seg_start) >> BLOCK_LOG] = (rel << 8) | (first - B);
+            B += BLOCK;
         }
         r += n - ks;
-        const uint32_t Llast = __shfl(L, (int)(n - 1));
-        c += 62ull * (n - 1) + Llast;
+        c = last + (uint64_t)__builtin_amdgcn_readlane(L, (int)(n - 1));
+        if (c > seg_end + 4096) {
+            if (lane == 0) {
+                out->y = ~0ull;
+                out->total = 0;
+            }
+            return;
+        }
     }
 }
 
@@ -885,29 +900,34 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
         uint64_t idx = first;
         uint32_t headN = 0, tailN = 0, headI = 0, tailI = 0;
         const uint64_t lt = (1ull << lane) - 1ull;
+        uint64_t lane_step = (uint64_t)(62u * (uint32_t)lane) * GAMMA;
+        asm volatile("" : "+v"(lane_step));  // opaque: keeps (c + 3) gamma + lane_step from being refactored
         // one step of the chain walk: up to 64 candidate starts into the rings
         auto enqueue = [&]() {
-            // speculate: lane j starts at c + 62 j (no immigrant, no jump before it)
-            const uint64_t cj = c + 62ull * (uint64_t)lane;
-            const bool im = draw_u24(st.seed + (cj + 3) * GAMMA) < a.immT;
-            const uint64_t pj = cj + (im ? 33u : 62u);
-            const bool jp = draw_u24(st.seed + pj * GAMMA) < a.jumpT;
+            // speculate: lane j starts at c + 62 j (no immigrant, no jump
+            // before it).  c, idx and n are the same in every lane (scalar
+            // registers); lane j's counter for draw c + 62 j + 3 is
+            // (seed + (c + 3) gamma) + 62 j gamma, one 64-bit add per lane.
+            const uint64_t x3 = st.seed + (c + 3) * GAMMA + lane_step;
+            const bool im = draw_u24(x3) < a.immT;
+            // the jump test: draw c + 62 j + 33 (immigrant) or + 62
+            const bool jp = draw_u24(x3 + (im ? 30ull * GAMMA : 59ull * GAMMA)) < a.jumpT;
             const uint32_t L = im ? (jp ? 35u : 33u) : (jp ? 64u : 62u);
-            const uint64_t stop = __ballot(L != 62u);
-            uint64_t n = stop ? (uint64_t)__ffsll((unsigned long long)stop) : 64ull;
-            const uint64_t room = (bend - c + 61) / 62;
+            const uint64_t stop = __ballot(im || jp);
+            uint32_t n = stop ? (uint32_t)__ffsll((unsigned long long)stop) : 64u;
+            const uint32_t room = (uint32_t)(bend - c + 61) / 62u;  // c < bend <= c + 2 BLOCK
             const uint64_t lam = st.my_lambda - idx;
-            n = min(n, min(room, lam));
-            const bool take = (uint64_t)lane < n;
-            const uint32_t code = (uint32_t)(cj - bstart) | ((uint32_t)(idx - first + (uint64_t)lane) << RUN_POS_BITS) |
+            n = (uint32_t)min((uint64_t)min(n, room), lam);
+            const bool take = (uint32_t)lane < n;
+            const uint32_t code = ((uint32_t)(c - bstart) + 62u * (uint32_t)lane) |
+                                  (((uint32_t)(idx - first) + (uint32_t)lane) << RUN_POS_BITS) |
                                   ((uint32_t)im << 30) | ((uint32_t)jp << 31);
             const uint64_t mN = __ballot(take && !im), mI = __ballot(take && im);
             if (take && !im) qN[(tailN + (uint32_t)__popcll(mN & lt)) & (QCAP - 1)] = code;
             if (take && im) qI[(tailI + (uint32_t)__popcll(mI & lt)) & (QCAP - 1)] = code;
             tailN += (uint32_t)__popcll(mN);
             tailI += (uint32_t)__popcll(mI);
-            const uint32_t Llast = __shfl(L, (int)(n - 1));
-            c += 62ull * (n - 1) + Llast;
+            c += 62ull * (n - 1) + (uint64_t)__builtin_amdgcn_readlane(L, (int)(n - 1));
             idx += n;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -1250,6 +1270,8 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
     std::vector<SegOut> so(nseg);
     HIPCHK(hipMemcpyAsync(so.data(), ctx->segout.p, sizeof(SegOut) * nseg, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (uint64_t f = 0; f < nseg; ++f)
+        if (so[f].y == ~0ull) return fail(CS_E_INTERNAL, "segment walk passed its end");
     // verify the chain segment by segment; re-walk segments that started on
     // another chain from the verified exit of their predecessor
     uint32_t fixes = 0;
@@ -1264,7 +1286,8 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
             HIPCHK(hipMemcpyAsync(&so[f], (SegOut *)ctx->segout.p + f, sizeof(SegOut), hipMemcpyDeviceToHost,
                                   ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));
-            if (so[f].x != so[f - 1].y) return fail(CS_E_INTERNAL, "segment fixup did not start on the chain");
+            if (so[f].x != so[f - 1].y || so[f].y == ~0ull)
+                return fail(CS_E_INTERNAL, "segment fixup did not start on the chain");
         }
     }
     std::vector<uint64_t> base(nseg, 0);
