@@ -123,9 +123,13 @@ extern "C" int cs_valu_rate_probe(int device, int op, uint32_t iters, double *ns
         hipFree(out);
         return fail(CS_E_OOM, "rate probe");
     }
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) hipEventDestroy(e0);
+        hipFree(out);
+        hipFree(cyc);
+        return fail(CS_E_HIP, "rate probe events");
+    }
     typedef void (*Kern)(uint32_t *, uint64_t *, uint32_t);
     static const Kern kerns[RATE_NOPS] = {k_rate<0>, k_rate<1>, k_rate<2>, k_rate<3>, k_rate<4>, k_rate<5>,
                                           k_rate<6>, k_rate<7>, k_rate<8>, k_rate<9>, k_rate<10>};

@@ -1533,7 +1533,15 @@ int cs_ctx_create(int device, cs_ctx **out) {
         delete c;
         return fail(CS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
-    for (auto &ev : c->ev) (void)hipEventCreate(&ev);
+    for (auto &ev : c->ev) {
+        if ((e = hipEventCreate(&ev)) != hipSuccess) {
+            for (auto &ev2 : c->ev)
+                if (ev2) (void)hipEventDestroy(ev2);
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return fail(CS_E_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+    }
     *out = c;
     return CS_OK;
 }
