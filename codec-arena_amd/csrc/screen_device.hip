@@ -1523,7 +1523,7 @@ int cs_ctx_create(int device, cs_ctx **out) {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-        return fail(CS_E_NODEV, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+        return fail(CS_E_NODEV, std::string("device is ") + prop.gcnArchName + ", libcsfunnel is compiled for gfx950 only");
     HIPCHK(hipSetDevice(device));
     cs_ctx *c = new cs_ctx();
     c->device = device;

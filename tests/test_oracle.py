@@ -328,5 +328,5 @@ def test_fmt_d4_matches_decimal_rounding_of_shortest_repr(oracle):
         from decimal import Decimal, ROUND_HALF_UP
         want = str(Decimal(short).quantize(Decimal("0.0001"), rounding=ROUND_HALF_UP))
         if Decimal(short) < Decimal("0.0001"):
-            continue  # below the first kept digit Zig's rounding differs
+            continue  # below 10^-4 Zig rounds on the leading digit, not at the 4th decimal
         assert oracle.fmt(float(v)) == want, (repr(v), short)
