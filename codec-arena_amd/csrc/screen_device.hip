@@ -1029,11 +1029,25 @@ struct HostState {
     double ms_score = 0, ms_replay = 0;
 };
 
-// Score the block ranges [lo[t], hi[t]) of every stream in one launch and
-// replay the emitted candidates through the stream heaps.  On record-buffer
-// overflow the ranges are split in two and processed in order.
-static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std::vector<uint64_t> &hi) {
+// A scoring launch uses one of two buffer sets (launch segments, record
+// buffer, counters, start/end events), so that the next launch can run while
+// the host replays this one's records.
+static LaunchSeg *buf_segs(cs_ctx *ctx, int T, int buf) { return (LaunchSeg *)ctx->segs.p + (size_t)buf * T; }
+static uint64_t *buf_seg_first(cs_ctx *ctx, int T, int buf) { return (uint64_t *)ctx->seg_first.p + (size_t)buf * (T + 1); }
+static Record *buf_rec(cs_ctx *ctx, int buf) { return (Record *)ctx->rec.p + (size_t)buf * ctx->rec_cap; }
+static char *buf_counters(cs_ctx *ctx, int buf) { return (char *)ctx->counters.p + 32 * buf; }
+static hipEvent_t buf_ev_start(cs_ctx *ctx, int buf) { return ctx->ev[buf ? 6 : 4]; }
+static hipEvent_t buf_ev_end(cs_ctx *ctx, int buf) { return ctx->ev[buf ? 7 : 5]; }
+
+// Launches the scoring of the block ranges [lo[t], hi[t]) of every stream
+// with buffer set `buf`; each stream's bound is its heap's root as replayed
+// so far (a bound from fewer candidates is lower, so the launch emits a
+// superset of the records the heap can take).  *launched = false for empty
+// ranges.
+static int launch_ranges(HostState &h, const std::vector<uint64_t> &lo, const std::vector<uint64_t> &hi, int buf,
+                         bool *launched) {
     cs_ctx *ctx = h.ctx;
+    *launched = false;
     const int T = h.T;
     std::vector<LaunchSeg> segs(T);
     std::vector<uint64_t> first(T + 1, 0);
@@ -1049,20 +1063,24 @@ static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std
     const uint64_t nflat = first[T];
     if (nflat == 0) return CS_OK;
     int rc;
-    if ((rc = ensure(ctx->segs, sizeof(LaunchSeg) * T))) return rc;
-    if ((rc = ensure(ctx->seg_first, sizeof(uint64_t) * (T + 1)))) return rc;
-    HIPCHK(hipMemcpyAsync(ctx->segs.p, segs.data(), sizeof(LaunchSeg) * T, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->seg_first.p, first.data(), sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice,
+    if ((rc = ensure(ctx->segs, sizeof(LaunchSeg) * T * 2))) return rc;
+    if ((rc = ensure(ctx->seg_first, sizeof(uint64_t) * (T + 1) * 2))) return rc;
+    HIPCHK(hipMemcpyAsync(buf_segs(ctx, T, buf), segs.data(), sizeof(LaunchSeg) * T, hipMemcpyHostToDevice,
                           ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->counters.p, 0, 16, ctx->stream));
+    HIPCHK(hipMemcpyAsync(buf_seg_first(ctx, T, buf), first.data(), sizeof(uint64_t) * (T + 1),
+                          hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(buf_counters(ctx, buf), 0, 16, ctx->stream));
     ScoreArgs a = h.sa;
-    a.segs = (const LaunchSeg *)ctx->segs.p;
-    a.seg_first = (const uint64_t *)ctx->seg_first.p;
+    a.segs = buf_segs(ctx, T, buf);
+    a.seg_first = buf_seg_first(ctx, T, buf);
     a.nflat = nflat;
+    a.rec = buf_rec(ctx, buf);
+    a.rec_count = (uint32_t *)buf_counters(ctx, buf);
+    a.cand_count = (unsigned long long *)(buf_counters(ctx, buf) + 8);
     const unsigned nw = h.sc == SC_QUAD ? score_waves<SC_QUAD>() : score_waves<SC_MLP_BF16>();
     uint64_t grid = (nflat + nw - 1) / nw;
     if (grid > (1u << 20)) grid = 1u << 20;
-    HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+    HIPCHK(hipEventRecord(buf_ev_start(ctx, buf), ctx->stream));
     if (h.sc != SC_QUAD) {
         // persistent: the workgroup loads the MLP weights into LDS once
         const uint64_t per_cu = std::max<uint64_t>(1, (160u * 1024u) / h.smem);
@@ -1094,15 +1112,32 @@ static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std
         go(k_score<false, false, SC_QUAD>);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
-    uint32_t cnt = 0;
-    unsigned long long cand = 0;
-    HIPCHK(hipMemcpyAsync(&cnt, a.rec_count, 4, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(&cand, a.cand_count, 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    h.ms_score += ms_between(ctx->ev[4], ctx->ev[5]);
+    HIPCHK(hipEventRecord(buf_ev_end(ctx, buf), ctx->stream));
+    *launched = true;
+    return CS_OK;
+}
+
+static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std::vector<uint64_t> &hi, int buf);
+
+// Waits for the launch of buffer set `buf` (the next launch may still run on
+// the stream), then replays its records through the stream heaps.  On
+// record-buffer overflow the ranges are split in two and scored again in
+// order with the same buffer set, before any later launch is replayed.
+static int finish_ranges(HostState &h, const std::vector<uint64_t> &lo, const std::vector<uint64_t> &hi, int buf) {
+    cs_ctx *ctx = h.ctx;
+    const int T = h.T;
+    int rc;
+    HIPCHK(hipEventSynchronize(buf_ev_end(ctx, buf)));
+    h.ms_score += ms_between(buf_ev_start(ctx, buf), buf_ev_end(ctx, buf));
     h.launches++;
     auto t0 = std::chrono::steady_clock::now();
+    // the stream is non-blocking: these copies do not wait for a later launch
+    unsigned char cbuf[16];
+    HIPCHK(hipMemcpy(cbuf, buf_counters(ctx, buf), 16, hipMemcpyDeviceToHost));
+    uint32_t cnt;
+    unsigned long long cand;
+    std::memcpy(&cnt, cbuf, 4);
+    std::memcpy(&cand, cbuf + 8, 8);
     if (cnt > ctx->rec_cap) {
         // overflow: split and redo in order (streams are independent)
         h.retries++;
@@ -1127,14 +1162,14 @@ static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std
             }
         }
         h.ms_replay += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        if ((rc = score_ranges(h, lo1, hi1))) return rc;
-        return score_ranges(h, lo2, hi2);
+        if ((rc = score_ranges(h, lo1, hi1, buf))) return rc;
+        return score_ranges(h, lo2, hi2, buf);
     }
     h.records += cnt;
     h.cand += cand;
     if (cnt) {
         if (ctx->host_rec.size() < cnt) ctx->host_rec.resize(cnt);
-        HIPCHK(hipMemcpy(ctx->host_rec.data(), a.rec, sizeof(Record) * cnt, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ctx->host_rec.data(), buf_rec(ctx, buf), sizeof(Record) * cnt, hipMemcpyDeviceToHost));
         // replay order (stream, candidate index): sort 16-byte keys, not the records
         std::vector<std::pair<uint64_t, uint64_t>> order(cnt);
         for (uint32_t i = 0; i < cnt; ++i)
@@ -1159,6 +1194,14 @@ static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std
     }
     h.ms_replay += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return CS_OK;
+}
+
+// One launch and its replay, in order.
+static int score_ranges(HostState &h, const std::vector<uint64_t> &lo, const std::vector<uint64_t> &hi, int buf) {
+    bool launched;
+    int rc;
+    if ((rc = launch_ranges(h, lo, hi, buf, &launched))) return rc;
+    return launched ? finish_ranges(h, lo, hi, buf) : CS_OK;
 }
 
 // Exact offsets from transfer maps of every block (any rates).
@@ -1467,7 +1510,15 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
                  nw * MLP_WAVE_BYTES;
     }
     {
-        std::vector<uint64_t> lo(T), hi(T);
+        // The quadratic screen launches level k + 1 before it replays level
+        // k's records, so the replay runs beside the next launch; level k + 1
+        // then filters with the bound of the levels before k, which is lower
+        // and so still passes every candidate the heaps can take.
+        const bool overlap = sc == SC_QUAD && h.sa.dump_genes == nullptr;
+        if (overlap && (rc = ensure(ctx->rec, sizeof(Record) * ctx->rec_cap * 2))) return rc;
+        std::vector<uint64_t> lo(T), hi(T), plo, phi;
+        bool pending = false;
+        int buf = 0, pbuf = 0;
         uint64_t maxnb = 0;
         for (int t = 0; t < T; ++t) maxnb = std::max(maxnb, h.si[t].nblocks);
         uint64_t a0 = 0, a1 = 1;
@@ -1476,10 +1527,24 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
                 lo[t] = std::min(a0, h.si[t].nblocks);
                 hi[t] = std::min(a1, h.si[t].nblocks);
             }
-            if ((rc = score_ranges(h, lo, hi))) return rc;
+            if (overlap) {
+                bool launched;
+                if ((rc = launch_ranges(h, lo, hi, buf, &launched))) return rc;
+                if (pending && (rc = finish_ranges(h, plo, phi, pbuf))) return rc;
+                pending = launched;
+                if (launched) {
+                    plo = lo;
+                    phi = hi;
+                    pbuf = buf;
+                    buf ^= 1;
+                }
+            } else if ((rc = score_ranges(h, lo, hi, 0))) {
+                return rc;
+            }
             a0 = a1;
             a1 *= 4;
         }
+        if (pending && (rc = finish_ranges(h, plo, phi, pbuf))) return rc;
     }
     // ---- 4. merge heaps t-major, stable sort by score descending
     auto tm0 = std::chrono::steady_clock::now();
