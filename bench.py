@@ -22,7 +22,6 @@ generations whose survivors the codec label source re-scores.
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
