@@ -73,6 +73,7 @@ SIGNATURES = {
                             C.POINTER(ScreenCfg), C.POINTER(Survivor), C.c_uint64,
                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "cs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ScreenStats)]),
+    "cs_set_record_cap": (C.c_int, [C.c_void_p, C.c_uint32]),
     "cs_set_offset_mode": (C.c_int, [C.c_void_p, C.c_int]),
     "cs_mlp_create": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "cs_mlp_destroy": (C.c_int, [C.c_void_p]),
@@ -95,6 +96,7 @@ SIGNATURES = {
                                          C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]),
     "cs_format_gene": (C.c_int, [C.c_float, C.c_char_p, C.c_size_t]),
     "cs_write_promoted": (C.c_int, [C.c_char_p, C.POINTER(Survivor), C.c_uint64]),
+    "cs_write_promoted_v6": (C.c_int, [C.c_char_p, C.POINTER(Survivor), C.c_uint64]),
     "cs_write_count": (C.c_int, [C.c_char_p, C.c_uint64]),
     "cs_valu_rate_probe": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_uint32)]),

@@ -12,6 +12,7 @@ module only marshals arguments.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import weakref
 import sys
 from dataclasses import dataclass
@@ -173,6 +174,10 @@ class Screen:
         """0 = automatic (speculative walk, verified), 1 = transfer maps."""
         check(lib().cs_set_offset_mode(self._h, int(mode)))
 
+    def set_record_cap(self, cap: int) -> None:
+        """Records per scoring-launch buffer (testing hook; 2048 <= cap <= 2^20)."""
+        check(lib().cs_set_record_cap(self._h, int(cap)))
+
     def stats(self) -> dict:
         s = _abi.ScreenStats()
         check(lib().cs_get_stats(self._h, C.byref(s)))
@@ -182,6 +187,13 @@ class Screen:
 def write_promoted(path: str, survivors: np.ndarray) -> None:
     arr = np.ascontiguousarray(survivors, dtype=SURVIVOR_DTYPE)
     check(lib().cs_write_promoted(path.encode(), arr.ctypes.data_as(C.POINTER(Survivor)), len(arr)))
+
+
+def write_promoted_v6(path: str, survivors: np.ndarray) -> None:
+    """The survivor lines with all 29 genes (34 tokens): the lines
+    brain/evolve.py:445-454 accepts, unlike the reference's 29-token lines."""
+    arr = np.ascontiguousarray(survivors, dtype=SURVIVOR_DTYPE)
+    check(lib().cs_write_promoted_v6(path.encode(), arr.ctypes.data_as(C.POINTER(Survivor)), len(arr)))
 
 
 def write_count(path: str, total: int) -> None:
@@ -218,11 +230,21 @@ def format_gene(v: float) -> str:
     return buf.value.decode()
 
 
+V6_ENV = "CSFUNNEL_PROMOTED_FORMAT"
+
+
 def screen_cmd(argv: List[str], device: int = 0) -> int:
     """`arena screen` (src/main.zig:156-218): flag parsing and defaults of the
     reference, model + elites loading, the screen itself, both output files and
-    the stderr summary.  Returns the process exit code (1 on any error)."""
+    the stderr summary.  Returns the process exit code (1 on any error).
+
+    Two additions, both off by default so the output files stay byte-identical
+    to the reference's: `--v6-out FILE` also writes the survivors as 34-token
+    lines with all 29 genes, and CSFUNNEL_PROMOTED_FORMAT=v6 in the
+    environment writes `--out` itself that way -- the form brain/evolve.py
+    (which passes no extra flags) keeps instead of dropping every line."""
     model_path = "state/surrogate.txt"
+    v6_out = None
     elites_path = "out/elites.txt"
     out_path = "out/promoted.txt"
     count_out = "out/screens.txt"
@@ -267,14 +289,22 @@ def screen_cmd(argv: List[str], device: int = 0) -> int:
             elif arg == "--imm":
                 i += 1
                 cfg.imm_p = parse_f32(a[i])
+            elif arg == "--v6-out":
+                i += 1
+                v6_out = a[i]
             i += 1
+        fmt = os.environ.get(V6_ENV, "")
+        if fmt not in ("", "zig", "v6"):
+            raise CsError(-3, f"{V6_ENV} must be 'zig' or 'v6', got {fmt!r}")
         model = load_model(model_path)
         elites = load_elites(elites_path)
         if len(elites) < 2:
             raise CsError(-2, "fewer than two elites")
         with Screen(device) as sc:
             surv, total = sc.run(model, elites, cfg)
-        write_promoted(out_path, surv)
+        (write_promoted_v6 if fmt == "v6" else write_promoted)(out_path, surv)
+        if v6_out is not None:
+            write_promoted_v6(v6_out, surv)
         write_count(count_out, total)
     except CsError as e:
         sys.stderr.write(f"error: {e.name}\n{e}\n")

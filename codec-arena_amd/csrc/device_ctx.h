@@ -37,14 +37,21 @@ inline int ensure(DevBuf &b, size_t bytes) {
     return CS_OK;
 }
 
+// Device buffers of one screen call.  A context owns one set and grows it
+// with hipMalloc; the allocation-free entry point carves a set out of the
+// library's static scratch array instead.
+struct ScreenBufs {
+    DevBuf streams, blk_first, maps, entries, upper_maps, upper_entries, level_first, elites, segs, seg_first, rec,
+        counters, model, seg_first_buf, segout, seg_base;
+};
+
 }  // namespace csf
 
 struct cs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
-    csf::DevBuf streams, blk_first, maps, entries, upper_maps, upper_entries, level_first, elites,
-        segs, seg_first, rec, counters, model, seg_first_buf, segout, seg_base;
+    csf::ScreenBufs sb;
     csf::DevBuf gram_part, ridge_part, ridge_scalar;
     std::vector<csf::Record> host_rec;
     cs_screen_stats stats{};

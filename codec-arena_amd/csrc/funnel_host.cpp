@@ -552,9 +552,12 @@ int cs_format_gene(float v, char *buf, size_t cap) {
     return (int)s.size();
 }
 
-int cs_write_promoted(const char *path, const cs_survivor *s, uint64_t n) {
+}  // extern "C"
+
+// "0 0 pa pb tribe" and the first `ngenes` genes at {d:.4}, one line per survivor.
+static int write_survivor_lines(const char *path, const cs_survivor *s, uint64_t n, int ngenes) {
     std::string txt;
-    txt.reserve(n * 200);
+    txt.reserve(n * 220);
     for (uint64_t i = 0; i < n; ++i) {
         txt += "0 0 ";
         txt += std::to_string(s[i].pa);
@@ -562,7 +565,7 @@ int cs_write_promoted(const char *path, const cs_survivor *s, uint64_t n) {
         txt += std::to_string(s[i].pb);
         txt += ' ';
         txt += std::to_string(s[i].tribe);
-        for (int k = 0; k < 24; ++k) {
+        for (int k = 0; k < ngenes; ++k) {
             txt += ' ';
             format_d4(s[i].g[k], txt);
         }
@@ -573,6 +576,17 @@ int cs_write_promoted(const char *path, const cs_survivor *s, uint64_t n) {
     size_t w = std::fwrite(txt.data(), 1, txt.size(), f);
     std::fclose(f);
     return w == txt.size() ? CS_OK : fail(CS_E_IO, "short write");
+}
+
+extern "C" {
+
+// src/screen.zig:326-337 writes 24 genes (29 tokens per line)
+int cs_write_promoted(const char *path, const cs_survivor *s, uint64_t n) {
+    return write_survivor_lines(path, s, n, 24);
+}
+
+int cs_write_promoted_v6(const char *path, const cs_survivor *s, uint64_t n) {
+    return write_survivor_lines(path, s, n, CS_NTOT);
 }
 
 int cs_write_count(const char *path, uint64_t total) {

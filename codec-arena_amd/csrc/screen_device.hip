@@ -1013,6 +1013,10 @@ static float nan_f() { return std::nanf(""); }
 
 struct HostState {
     cs_ctx *ctx;
+    ScreenBufs *B = nullptr;    // device buffers of this call
+    char *arena = nullptr;      // non-null: carve B out of this scratch memory
+    size_t arena_used = 0, arena_cap = 0;
+    uint32_t rec_cap = 0;       // records per record buffer
     int T;
     std::vector<StreamInfo> si;
     std::vector<StreamHeap> heaps;
@@ -1029,13 +1033,30 @@ struct HostState {
     double ms_score = 0, ms_replay = 0;
 };
 
+// Device memory for buffer b of this call: the context's buffer, grown with
+// hipMalloc, or (allocation-free calls) the next 256-byte aligned piece of the
+// scratch array.  A scratch piece already carved and large enough is reused.
+static int acquire(HostState &h, DevBuf &b, size_t bytes) {
+    if (!h.arena) return ensure(b, bytes);
+    if (b.p && b.bytes >= bytes) return CS_OK;
+    const size_t off = (h.arena_used + 255) & ~(size_t)255;
+    if (off > h.arena_cap || bytes > h.arena_cap - off)
+        return fail(CS_E_CAP, "cs_screen_noalloc: this call needs more than the " +
+                                  std::to_string(h.arena_cap >> 20) + " MiB scratch array (at least " +
+                                  std::to_string((off + bytes + (1 << 20) - 1) >> 20) + " MiB)");
+    b.p = h.arena + off;
+    b.bytes = bytes;
+    h.arena_used = off + bytes;
+    return CS_OK;
+}
+
 // A scoring launch uses one of two buffer sets (launch segments, record
 // buffer, counters, start/end events), so that the next launch can run while
 // the host replays this one's records.
-static LaunchSeg *buf_segs(cs_ctx *ctx, int T, int buf) { return (LaunchSeg *)ctx->segs.p + (size_t)buf * T; }
-static uint64_t *buf_seg_first(cs_ctx *ctx, int T, int buf) { return (uint64_t *)ctx->seg_first.p + (size_t)buf * (T + 1); }
-static Record *buf_rec(cs_ctx *ctx, int buf) { return (Record *)ctx->rec.p + (size_t)buf * ctx->rec_cap; }
-static char *buf_counters(cs_ctx *ctx, int buf) { return (char *)ctx->counters.p + 32 * buf; }
+static LaunchSeg *buf_segs(HostState &h, int buf) { return (LaunchSeg *)h.B->segs.p + (size_t)buf * h.T; }
+static uint64_t *buf_seg_first(HostState &h, int buf) { return (uint64_t *)h.B->seg_first.p + (size_t)buf * (h.T + 1); }
+static Record *buf_rec(HostState &h, int buf) { return (Record *)h.B->rec.p + (size_t)buf * h.rec_cap; }
+static char *buf_counters(HostState &h, int buf) { return (char *)h.B->counters.p + 32 * buf; }
 static hipEvent_t buf_ev_start(cs_ctx *ctx, int buf) { return ctx->ev[buf ? 6 : 4]; }
 static hipEvent_t buf_ev_end(cs_ctx *ctx, int buf) { return ctx->ev[buf ? 7 : 5]; }
 
@@ -1063,20 +1084,21 @@ static int launch_ranges(HostState &h, const std::vector<uint64_t> &lo, const st
     const uint64_t nflat = first[T];
     if (nflat == 0) return CS_OK;
     int rc;
-    if ((rc = ensure(ctx->segs, sizeof(LaunchSeg) * T * 2))) return rc;
-    if ((rc = ensure(ctx->seg_first, sizeof(uint64_t) * (T + 1) * 2))) return rc;
-    HIPCHK(hipMemcpyAsync(buf_segs(ctx, T, buf), segs.data(), sizeof(LaunchSeg) * T, hipMemcpyHostToDevice,
+    if ((rc = acquire(h, h.B->segs, sizeof(LaunchSeg) * T * 2))) return rc;
+    if ((rc = acquire(h, h.B->seg_first, sizeof(uint64_t) * (T + 1) * 2))) return rc;
+    HIPCHK(hipMemcpyAsync(buf_segs(h, buf), segs.data(), sizeof(LaunchSeg) * T, hipMemcpyHostToDevice,
                           ctx->stream));
-    HIPCHK(hipMemcpyAsync(buf_seg_first(ctx, T, buf), first.data(), sizeof(uint64_t) * (T + 1),
+    HIPCHK(hipMemcpyAsync(buf_seg_first(h, buf), first.data(), sizeof(uint64_t) * (T + 1),
                           hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemsetAsync(buf_counters(ctx, buf), 0, 16, ctx->stream));
+    HIPCHK(hipMemsetAsync(buf_counters(h, buf), 0, 16, ctx->stream));
     ScoreArgs a = h.sa;
-    a.segs = buf_segs(ctx, T, buf);
-    a.seg_first = buf_seg_first(ctx, T, buf);
+    a.segs = buf_segs(h, buf);
+    a.seg_first = buf_seg_first(h, buf);
     a.nflat = nflat;
-    a.rec = buf_rec(ctx, buf);
-    a.rec_count = (uint32_t *)buf_counters(ctx, buf);
-    a.cand_count = (unsigned long long *)(buf_counters(ctx, buf) + 8);
+    a.rec = buf_rec(h, buf);
+    a.rec_cap = h.rec_cap;
+    a.rec_count = (uint32_t *)buf_counters(h, buf);
+    a.cand_count = (unsigned long long *)(buf_counters(h, buf) + 8);
     const unsigned nw = h.sc == SC_QUAD ? score_waves<SC_QUAD>() : score_waves<SC_MLP_BF16>();
     uint64_t grid = (nflat + nw - 1) / nw;
     if (grid > (1u << 20)) grid = 1u << 20;
@@ -1133,12 +1155,12 @@ static int finish_ranges(HostState &h, const std::vector<uint64_t> &lo, const st
     auto t0 = std::chrono::steady_clock::now();
     // the stream is non-blocking: these copies do not wait for a later launch
     unsigned char cbuf[16];
-    HIPCHK(hipMemcpy(cbuf, buf_counters(ctx, buf), 16, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cbuf, buf_counters(h, buf), 16, hipMemcpyDeviceToHost));
     uint32_t cnt;
     unsigned long long cand;
     std::memcpy(&cnt, cbuf, 4);
     std::memcpy(&cand, cbuf + 8, 8);
-    if (cnt > ctx->rec_cap) {
+    if (cnt > h.rec_cap) {
         // overflow: split and redo in order (streams are independent)
         h.retries++;
         std::vector<uint64_t> lo1 = lo, hi1 = hi, lo2 = lo, hi2 = hi;
@@ -1169,7 +1191,7 @@ static int finish_ranges(HostState &h, const std::vector<uint64_t> &lo, const st
     h.cand += cand;
     if (cnt) {
         if (ctx->host_rec.size() < cnt) ctx->host_rec.resize(cnt);
-        HIPCHK(hipMemcpy(ctx->host_rec.data(), buf_rec(ctx, buf), sizeof(Record) * cnt, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ctx->host_rec.data(), buf_rec(h, buf), sizeof(Record) * cnt, hipMemcpyDeviceToHost));
         // replay order (stream, candidate index): sort 16-byte keys, not the records
         std::vector<std::pair<uint64_t, uint64_t>> order(cnt);
         for (uint32_t i = 0; i < cnt; ++i)
@@ -1209,14 +1231,14 @@ static int offsets_by_maps(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
                            uint32_t immT, uint32_t jumpT) {
     const int T = h.T;
     int rc;
-    if ((rc = ensure(ctx->maps, sizeof(uint64_t) * 64 * nblk))) return rc;
+    if ((rc = acquire(h, h.B->maps, sizeof(uint64_t) * 64 * nblk))) return rc;
     // ---- 1. block maps
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     {
         uint64_t grid = std::min<uint64_t>(nblk, 1u << 22);
         hipLaunchKernelGGL(k_map, dim3((unsigned)grid), dim3(256), 0, ctx->stream,
-                           (const StreamInfo *)ctx->streams.p, (const uint64_t *)ctx->blk_first.p, T, nblk, immT,
-                           jumpT, (uint64_t *)ctx->maps.p);
+                           (const StreamInfo *)h.B->streams.p, (const uint64_t *)h.B->blk_first.p, T, nblk, immT,
+                           jumpT, (uint64_t *)h.B->maps.p);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -1238,21 +1260,21 @@ static int offsets_by_maps(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
         lvl_first.push_back(nx);
     }
     const int L = (int)lvl_first.size() - 1;
-    if ((rc = ensure(ctx->upper_maps, sizeof(uint64_t) * 64 * (upper_total + 1)))) return rc;
-    if ((rc = ensure(ctx->upper_entries, sizeof(uint64_t) * (upper_total + 1)))) return rc;
-    if ((rc = ensure(ctx->level_first, sizeof(uint64_t) * (T + 1) * (L + 1)))) return rc;
+    if ((rc = acquire(h, h.B->upper_maps, sizeof(uint64_t) * 64 * (upper_total + 1)))) return rc;
+    if ((rc = acquire(h, h.B->upper_entries, sizeof(uint64_t) * (upper_total + 1)))) return rc;
+    if ((rc = acquire(h, h.B->level_first, sizeof(uint64_t) * (T + 1) * (L + 1)))) return rc;
     // host staging vectors stay alive until the stream synchronizes below
     std::vector<uint64_t> flat_first;
     for (auto &v : lvl_first) flat_first.insert(flat_first.end(), v.begin(), v.end());
-    HIPCHK(hipMemcpyAsync(ctx->level_first.p, flat_first.data(), flat_first.size() * 8, hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyAsync(h.B->level_first.p, flat_first.data(), flat_first.size() * 8, hipMemcpyHostToDevice,
                           ctx->stream));
     auto lvl_maps = [&](int l) -> uint64_t * {
-        return l == 0 ? (uint64_t *)ctx->maps.p : (uint64_t *)ctx->upper_maps.p + lvl_off[l - 1] * 64;
+        return l == 0 ? (uint64_t *)h.B->maps.p : (uint64_t *)h.B->upper_maps.p + lvl_off[l - 1] * 64;
     };
     auto lvl_entries = [&](int l) -> uint64_t * {
-        return l == 0 ? (uint64_t *)ctx->entries.p : (uint64_t *)ctx->upper_entries.p + lvl_off[l - 1];
+        return l == 0 ? (uint64_t *)h.B->entries.p : (uint64_t *)h.B->upper_entries.p + lvl_off[l - 1];
     };
-    auto lvl_firstp = [&](int l) -> const uint64_t * { return (const uint64_t *)ctx->level_first.p + (size_t)l * (T + 1); };
+    auto lvl_firstp = [&](int l) -> const uint64_t * { return (const uint64_t *)h.B->level_first.p + (size_t)l * (T + 1); };
     for (int l = 0; l < L; ++l) {
         const uint64_t ng = lvl_first[l + 1][T];
         unsigned grid = (unsigned)std::min<uint64_t>(ng, 1u << 20);
@@ -1297,21 +1319,21 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
         seg_first[t + 1] = seg_first[t] + ((nb + (1ull << SEG_BLOCKS_LOG) - 1) >> SEG_BLOCKS_LOG);
     }
     const uint64_t nseg = seg_first[T];
-    if ((rc = ensure(ctx->seg_first_buf, sizeof(uint64_t) * (T + 1)))) return rc;
-    if ((rc = ensure(ctx->segout, sizeof(SegOut) * nseg))) return rc;
-    if ((rc = ensure(ctx->seg_base, sizeof(uint64_t) * nseg))) return rc;
-    HIPCHK(hipMemcpyAsync(ctx->seg_first_buf.p, seg_first.data(), 8 * (T + 1), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = acquire(h, h.B->seg_first_buf, sizeof(uint64_t) * (T + 1)))) return rc;
+    if ((rc = acquire(h, h.B->segout, sizeof(SegOut) * nseg))) return rc;
+    if ((rc = acquire(h, h.B->seg_base, sizeof(uint64_t) * nseg))) return rc;
+    HIPCHK(hipMemcpyAsync(h.B->seg_first_buf.p, seg_first.data(), 8 * (T + 1), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     {
         unsigned grid = (unsigned)std::min<uint64_t>(nseg, 1u << 20);
-        hipLaunchKernelGGL(k_walk, dim3(grid), dim3(64), 0, ctx->stream, (const StreamInfo *)ctx->streams.p,
-                           (const uint64_t *)ctx->seg_first_buf.p, T, nseg, immT, jumpT, (uint64_t *)ctx->entries.p,
-                           (SegOut *)ctx->segout.p, (int64_t)-1, (uint64_t)0);
+        hipLaunchKernelGGL(k_walk, dim3(grid), dim3(64), 0, ctx->stream, (const StreamInfo *)h.B->streams.p,
+                           (const uint64_t *)h.B->seg_first_buf.p, T, nseg, immT, jumpT, (uint64_t *)h.B->entries.p,
+                           (SegOut *)h.B->segout.p, (int64_t)-1, (uint64_t)0);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
     std::vector<SegOut> so(nseg);
-    HIPCHK(hipMemcpyAsync(so.data(), ctx->segout.p, sizeof(SegOut) * nseg, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(so.data(), h.B->segout.p, sizeof(SegOut) * nseg, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (uint64_t f = 0; f < nseg; ++f)
         if (so[f].y == ~0ull) return fail(CS_E_INTERNAL, "segment walk passed its end");
@@ -1322,11 +1344,11 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
         for (uint64_t f = seg_first[t] + 1; f < seg_first[t + 1]; ++f) {
             if (so[f].x == so[f - 1].y) continue;
             if (++fixes > 32) return CS_OK;  // *ok stays false
-            hipLaunchKernelGGL(k_walk, dim3(1), dim3(64), 0, ctx->stream, (const StreamInfo *)ctx->streams.p,
-                               (const uint64_t *)ctx->seg_first_buf.p, T, (uint64_t)1, immT, jumpT,
-                               (uint64_t *)ctx->entries.p, (SegOut *)ctx->segout.p, (int64_t)f, so[f - 1].y);
+            hipLaunchKernelGGL(k_walk, dim3(1), dim3(64), 0, ctx->stream, (const StreamInfo *)h.B->streams.p,
+                               (const uint64_t *)h.B->seg_first_buf.p, T, (uint64_t)1, immT, jumpT,
+                               (uint64_t *)h.B->entries.p, (SegOut *)h.B->segout.p, (int64_t)f, so[f - 1].y);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(&so[f], (SegOut *)ctx->segout.p + f, sizeof(SegOut), hipMemcpyDeviceToHost,
+            HIPCHK(hipMemcpyAsync(&so[f], (SegOut *)h.B->segout.p + f, sizeof(SegOut), hipMemcpyDeviceToHost,
                                   ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));
             if (so[f].x != so[f - 1].y || so[f].y == ~0ull)
@@ -1342,12 +1364,12 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
         }
         if (acc < h.si[t].my_lambda) return fail(CS_E_INTERNAL, "block budget did not cover the stream's candidates");
     }
-    HIPCHK(hipMemcpyAsync(ctx->seg_base.p, base.data(), 8 * nseg, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.B->seg_base.p, base.data(), 8 * nseg, hipMemcpyHostToDevice, ctx->stream));
     {
         unsigned grid = (unsigned)std::min<uint64_t>((nblk + 255) / 256, 1u << 16);
-        hipLaunchKernelGGL(k_rebase, dim3(grid), dim3(256), 0, ctx->stream, (const StreamInfo *)ctx->streams.p,
-                           (const uint64_t *)ctx->blk_first.p, T, nblk, (const uint64_t *)ctx->seg_first_buf.p,
-                           (const uint64_t *)ctx->seg_base.p, (uint64_t *)ctx->entries.p);
+        hipLaunchKernelGGL(k_rebase, dim3(grid), dim3(256), 0, ctx->stream, (const StreamInfo *)h.B->streams.p,
+                           (const uint64_t *)h.B->blk_first.p, T, nblk, (const uint64_t *)h.B->seg_first_buf.p,
+                           (const uint64_t *)h.B->seg_base.p, (uint64_t *)h.B->entries.p);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
@@ -1383,6 +1405,8 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     const int T = (int)T64;
     HostState h;
     h.ctx = ctx;
+    h.B = &ctx->sb;
+    h.rec_cap = ctx->rec_cap;
     h.T = T;
     h.elites = elites;
     h.si.resize(T);
@@ -1411,22 +1435,25 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     }
     blk_first[T] = nblk;
     int rc;
+    // The quadratic screen launches level k + 1 before it replays level k's
+    // records (two record buffers), see the scoring loop below.
+    const bool overlap = sc == SC_QUAD && (opts ? opts->dump_genes : nullptr) == nullptr;
     // ---- device buffers
-    if ((rc = ensure(ctx->streams, sizeof(StreamInfo) * T))) return rc;
-    if ((rc = ensure(ctx->blk_first, sizeof(uint64_t) * (T + 1)))) return rc;
-    if ((rc = ensure(ctx->entries, sizeof(uint64_t) * nblk))) return rc;
-    if ((rc = ensure(ctx->elites, sizeof(float) * CS_NTOT * (size_t)ne))) return rc;
-    if ((rc = ensure(ctx->rec, sizeof(Record) * ctx->rec_cap))) return rc;
-    if ((rc = ensure(ctx->counters, 64))) return rc;
+    if ((rc = acquire(h, h.B->streams, sizeof(StreamInfo) * T))) return rc;
+    if ((rc = acquire(h, h.B->blk_first, sizeof(uint64_t) * (T + 1)))) return rc;
+    if ((rc = acquire(h, h.B->entries, sizeof(uint64_t) * nblk))) return rc;
+    if ((rc = acquire(h, h.B->elites, sizeof(float) * CS_NTOT * (size_t)ne))) return rc;
+    if ((rc = acquire(h, h.B->rec, sizeof(Record) * h.rec_cap * (overlap ? 2 : 1)))) return rc;
+    if ((rc = acquire(h, h.B->counters, 64))) return rc;
     std::vector<float> eg((size_t)ne * CS_NTOT);
     for (int e = 0; e < ne; ++e) {
         for (int k = 0; k < CS_NG; ++k) eg[(size_t)e * CS_NTOT + k] = elites[e].g[k];
         for (int k = 0; k < CS_NR; ++k) eg[(size_t)e * CS_NTOT + CS_NG + k] = elites[e].reg[k];
     }
-    HIPCHK(hipMemcpyAsync(ctx->streams.p, h.si.data(), sizeof(StreamInfo) * T, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->blk_first.p, blk_first.data(), sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyAsync(h.B->streams.p, h.si.data(), sizeof(StreamInfo) * T, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.B->blk_first.p, blk_first.data(), sizeof(uint64_t) * (T + 1), hipMemcpyHostToDevice,
                           ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctx->elites.p, eg.data(), eg.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(h.B->elites.p, eg.data(), eg.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     // ---- 1-2. candidate offsets: speculative walk when the event rates make
     // chains merge quickly, the transfer maps otherwise (or on failure)
     const uint32_t walk_min = (uint32_t)(0.004 * 16777216.0);
@@ -1446,8 +1473,8 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         for (int i = 0; i < 16; ++i)
             for (int j = 0; j <= i; ++j) md.tri[i * (i + 1) / 2 + j] = model->quad[i * 16 + j];
     }
-    if ((rc = ensure(ctx->model, sizeof(DevModel)))) return rc;
-    HIPCHK(hipMemcpyAsync(ctx->model.p, &md, sizeof(DevModel), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = acquire(h, h.B->model, sizeof(DevModel)))) return rc;
+    HIPCHK(hipMemcpyAsync(h.B->model.p, &md, sizeof(DevModel), hipMemcpyHostToDevice, ctx->stream));
     h.sc = sc;
     h.sa.mlp = MlpScreenW{};
     h.sa.dump_genes = opts ? opts->dump_genes : nullptr;
@@ -1476,10 +1503,10 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         mw.q1 = sc == SC_MLP_FP8 ? GELU_Q / ml->h_scale : GELU_Q;
     }
     ScoreArgs &a = h.sa;
-    a.streams = (const StreamInfo *)ctx->streams.p;
+    a.streams = (const StreamInfo *)h.B->streams.p;
     a.T = T;
-    a.entries = (const uint64_t *)ctx->entries.p;
-    a.elites = (const float *)ctx->elites.p;
+    a.entries = (const uint64_t *)h.B->entries.p;
+    a.elites = (const float *)h.B->elites.p;
     a.ne = (uint32_t)ne;
     a.ne_magic = UINT64_MAX / (uint64_t)ne + 1;
     a.sigma = cfg->sigma;
@@ -1495,11 +1522,11 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     }
     a.immT = immT;
     a.jumpT = jumpT;
-    a.model = (const DevModel *)ctx->model.p;
-    a.rec = (Record *)ctx->rec.p;
-    a.rec_cap = ctx->rec_cap;
-    a.rec_count = (uint32_t *)ctx->counters.p;
-    a.cand_count = (unsigned long long *)((char *)ctx->counters.p + 8);
+    a.model = (const DevModel *)h.B->model.p;
+    a.rec = (Record *)h.B->rec.p;
+    a.rec_cap = h.rec_cap;
+    a.rec_count = (uint32_t *)h.B->counters.p;
+    a.cand_count = (unsigned long long *)((char *)h.B->counters.p + 8);
     const size_t el_bytes = (size_t)ne * CS_NTOT * 4;
     h.lds_elites = sc == SC_QUAD && el_bytes <= 48 * 1024 && h.sa.dump_genes == nullptr;
     if (sc == SC_QUAD) {
@@ -1514,8 +1541,6 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         // k's records, so the replay runs beside the next launch; level k + 1
         // then filters with the bound of the levels before k, which is lower
         // and so still passes every candidate the heaps can take.
-        const bool overlap = sc == SC_QUAD && h.sa.dump_genes == nullptr;
-        if (overlap && (rc = ensure(ctx->rec, sizeof(Record) * ctx->rec_cap * 2))) return rc;
         std::vector<uint64_t> lo(T), hi(T), plo, phi;
         bool pending = false;
         int buf = 0, pbuf = 0;
@@ -1616,9 +1641,10 @@ int cs_ctx_destroy(cs_ctx *c) {
     if (c->n_mlp > 0) return fail(CS_E_BADARGS, "context still has live MLP handles; destroy them first");
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    csf::DevBuf *bufs[] = {&c->streams, &c->blk_first, &c->maps, &c->entries, &c->upper_maps, &c->upper_entries,
-                           &c->level_first, &c->elites, &c->segs, &c->seg_first, &c->rec, &c->counters, &c->model,
-                           &c->seg_first_buf, &c->segout, &c->seg_base, &c->gram_part, &c->ridge_part,
+    csf::ScreenBufs &b = c->sb;
+    csf::DevBuf *bufs[] = {&b.streams, &b.blk_first, &b.maps, &b.entries, &b.upper_maps, &b.upper_entries,
+                           &b.level_first, &b.elites, &b.segs, &b.seg_first, &b.rec, &b.counters, &b.model,
+                           &b.seg_first_buf, &b.segout, &b.seg_base, &c->gram_part, &c->ridge_part,
                            &c->ridge_scalar};
     for (auto *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -1645,6 +1671,13 @@ int cs_screen_ex(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, in
     clear_error();
     HIPCHK(hipSetDevice(ctx->device));
     return screen_impl(ctx, model, elites, ne, cfg, opts, out, cap, n_out, screens);
+}
+
+int cs_set_record_cap(cs_ctx *ctx, uint32_t cap) {
+    if (!ctx || cap < 2048 || cap > (1u << 20))
+        return fail(CS_E_BADARGS, "record capacity must lie in [2048, 2^20]");
+    ctx->rec_cap = cap;
+    return CS_OK;
 }
 
 int cs_set_offset_mode(cs_ctx *ctx, int mode) {

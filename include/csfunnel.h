@@ -140,6 +140,11 @@ int cs_screen(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int n
 
 int cs_get_stats(cs_ctx *ctx, cs_screen_stats *out);
 
+/* Testing hook, no counterpart in the reference: records per record buffer
+ * of the scoring launches (default 2^20; cap in [2048, 2^20]).  A launch that
+ * emits more is split by streams, then by blocks, and redone. */
+int cs_set_record_cap(cs_ctx *ctx, uint32_t cap);
+
 /* How candidate offsets are recovered: 0 = automatic (speculative walk when
  * both event rates are >= 0.004, verified, else transfer maps), 1 = always
  * transfer maps.  Both give identical survivors; exposed for tests and
@@ -217,6 +222,12 @@ int cs_format_gene(float v, char *buf, size_t cap);
 
 /* src/screen.zig:322-337: "0 0 pa pb tribe" + 24 genes at {d:.4} per line. */
 int cs_write_promoted(const char *path, const cs_survivor *s, uint64_t n);
+
+/* No counterpart in the reference (its writer above drops genes 24..28, so
+ * brain/evolve.py:449, which keeps only 5 + 29 = 34-token lines, promotes
+ * none of them): the same lines with all 29 genes at {d:.4}, the v6 token
+ * count that src/genome.zig:116-188 and brain/evolve.py:445-454 read. */
+int cs_write_promoted_v6(const char *path, const cs_survivor *s, uint64_t n);
 
 /* src/screen.zig:338-341: "<total>\n". */
 int cs_write_count(const char *path, uint64_t total);

@@ -86,6 +86,8 @@ int or_model_from_compact(const float *w, int n, or_model *out);
 int or_load_model(const char *path, or_model *out);
 int or_load_elites(const char *path, or_genome *out, int cap, int *n_out);
 float or_score(const or_model *m, const float *g);
+float or_clamp01(float x);
+float or_score_simd(const or_model *m, const float *g);
 
 int or_synth(const or_genome *elites, int ne, uint64_t stream_seed, uint64_t start,
              float sigma, float jump_p, float imm_p, float *g_out, uint32_t *len_out,
@@ -93,6 +95,12 @@ int or_synth(const or_genome *elites, int ne, uint64_t stream_seed, uint64_t sta
 
 int or_screen(const or_model *m, const or_genome *elites, int ne, const or_cfg *cfg,
               int workers, or_slot *out, uint64_t cap, uint64_t *n_out, uint64_t *screens);
+
+int or_screen_simd(const or_model *m, const or_genome *elites, int ne, const or_cfg *cfg,
+                   int workers, or_slot *out, uint64_t cap, uint64_t *n_out, uint64_t *screens);
+
+int or_screen_chunked(const or_model *m, const or_genome *elites, int ne, const or_cfg *cfg, int workers,
+                      int chunks, or_slot *out, uint64_t cap, uint64_t *n_out, uint64_t *screens);
 
 int or_fmt_d4(float v, char *buf, size_t cap);
 int or_write_promoted(const char *path, const or_slot *s, uint64_t n);

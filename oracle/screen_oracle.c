@@ -23,6 +23,7 @@
 #include "oracle.h"
 
 #include <errno.h>
+#include <immintrin.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdio.h>
@@ -33,8 +34,9 @@
 
 /* ---------------------------------------------------------------- RNG --- */
 
-/* src/screen.zig:202-208 */
-uint64_t or_splitmix_next(uint64_t *state) {
+/* src/screen.zig:202-208 (inlined into the worker loop, as in the
+ * reference's ReleaseFast build; or_splitmix_next exports it) */
+static inline uint64_t sm_next(uint64_t *state) {
     *state += GAMMA;
     uint64_t z = *state;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -42,14 +44,16 @@ uint64_t or_splitmix_next(uint64_t *state) {
     return z ^ (z >> 31);
 }
 
+uint64_t or_splitmix_next(uint64_t *state) { return sm_next(state); }
+
 void or_splitmix_fill(uint64_t seed, uint64_t n, uint64_t *out) {
     uint64_t s = seed;
-    for (uint64_t i = 0; i < n; i++) out[i] = or_splitmix_next(&s);
+    for (uint64_t i = 0; i < n; i++) out[i] = sm_next(&s);
 }
 
 /* src/screen.zig:210-212: top 24 bits times 2^-24, exact in f32 */
 static inline float unit_of(uint64_t *s) {
-    return (float)(or_splitmix_next(s) >> 40) * (1.0f / 16777216.0f);
+    return (float)(sm_next(s) >> 40) * (1.0f / 16777216.0f);
 }
 
 /* src/screen.zig:284 */
@@ -320,7 +324,7 @@ int or_load_elites(const char *path, or_genome *out, int cap, int *n_out) {
 /* src/screen.zig:97-112, no contraction.  Lane j:
  *   acc_j = w0 + lin_j*g_j ; sum_j = ((0 + q[0][j]*(g_j*g_0)) + ...) ;
  *   score = ((lane_0 + lane_1) + ... + lane_15) + (0 + lin_16*g_16 + ... + lin_28*g_28) */
-float or_score(const or_model *m, const float *g) {
+static inline float score_scalar(const or_model *m, const float *g) {
     float lane[16];
     for (int j = 0; j < 16; j++) {
         float acc = m->w0 + m->lin[j] * g[j];
@@ -335,7 +339,49 @@ float or_score(const or_model *m, const float *g) {
     return s + r;
 }
 
-static inline float clamp01(float x) { return fmaxf(0.0f, fminf(1.0f, x)); }
+/* The same function in the form src/screen.zig:97-112 writes it: 16-wide
+ * vectors over the lane index j (GCC vector extensions for @Vector(16, f32);
+ * each lane's operations and their order are those of or_score, with no
+ * contraction), then the ordered lane sum and the linear tail. */
+typedef float v16f __attribute__((vector_size(64)));
+
+#define SPLAT16(x) ((v16f){(x), (x), (x), (x), (x), (x), (x), (x), (x), (x), (x), (x), (x), (x), (x), (x)})
+
+static inline float score_simd(const or_model *m, const float *g) {
+    v16f gv, lv, wv;
+    memcpy(&gv, g, 64);
+    memcpy(&lv, m->lin, 64);
+    v16f acc = SPLAT16(m->w0);
+    acc = acc + lv * gv;
+    v16f sum = SPLAT16(0.0f);
+    for (int i = 0; i < 16; i++) {
+        const v16f row = gv * SPLAT16(g[i]);
+        memcpy(&wv, m->quad + i * 16, 64);
+        sum = sum + wv * row;
+    }
+    const v16f lane = acc + sum;
+    float s = -0.0f;
+    for (int j = 0; j < 16; j++) s = s + lane[j];
+    float r = 0.0f;
+    for (int i = 16; i < OR_NTOT; i++) r = r + m->lin[i] * g[i];
+    return s + r;
+}
+
+/* src/genome.zig:292-294: @max(0.0, @min(1.0, x)), i.e. LLVM maxnum/minnum
+ * with constant non-NaN operands: a NaN gives the constant (clamp01(NaN) = 1)
+ * and an equal pair gives the constant, so clamp01(-0) = +0 (LLVM allows
+ * either zero there; the sign of that zero is not pinned by the reference). */
+float or_score(const or_model *m, const float *g) { return score_scalar(m, g); }
+float or_score_simd(const or_model *m, const float *g) { return score_simd(m, g); }
+
+static inline float clamp01(float x) {
+    /* minss(a, b) = a < b ? a : b and maxss(a, b) = a > b ? a : b (the
+     * second operand when unordered or equal): branch-free on x86 */
+    const __m128 t = _mm_min_ss(_mm_set_ss(x), _mm_set_ss(1.0f));
+    return _mm_cvtss_f32(_mm_max_ss(t, _mm_set_ss(0.0f)));
+}
+
+float or_clamp01(float x) { return clamp01(x); }
 
 /* One candidate of src/screen.zig:232-249 from draw counter `start`.
  * Returns the genome, the number of draws consumed and the elite indices. */
@@ -344,8 +390,8 @@ int or_synth(const or_genome *elites, int ne, uint64_t stream_seed, uint64_t sta
              uint32_t *ib_out) {
     uint64_t s = stream_seed + start * GAMMA;
     uint32_t used = 3;
-    uint32_t ia = (uint32_t)((or_splitmix_next(&s) >> 32) % (uint64_t)ne);
-    uint32_t ib = (uint32_t)((or_splitmix_next(&s) >> 32) % (uint64_t)ne);
+    uint32_t ia = (uint32_t)((sm_next(&s) >> 32) % (uint64_t)ne);
+    uint32_t ib = (uint32_t)((sm_next(&s) >> 32) % (uint64_t)ne);
     const or_genome *ea = &elites[ia], *eb = &elites[ib];
     if (unit_of(&s) < imm_p) {
         for (int k = 0; k < OR_NTOT; k++) g[k] = unit_of(&s);
@@ -354,7 +400,7 @@ int or_synth(const or_genome *elites, int ne, uint64_t stream_seed, uint64_t sta
         for (int k = 0; k < OR_NTOT; k++) {
             float pa = k < OR_NG ? ea->g[k] : ea->reg[k - OR_NG];
             float pb = k < OR_NG ? eb->g[k] : eb->reg[k - OR_NG];
-            float v = (or_splitmix_next(&s) & 1) == 0 ? pa : pb;
+            float v = (sm_next(&s) & 1) == 0 ? pa : pb;
             float u = unit_of(&s);
             v = v + ((u - 0.5f) * 2.0f) * sigma;
             g[k] = clamp01(v);
@@ -363,7 +409,7 @@ int or_synth(const or_genome *elites, int ne, uint64_t stream_seed, uint64_t sta
     }
     used += 1;
     if (unit_of(&s) < jump_p) {
-        uint32_t idx = (uint32_t)((or_splitmix_next(&s) >> 32) % (uint64_t)OR_NTOT);
+        uint32_t idx = (uint32_t)((sm_next(&s) >> 32) % (uint64_t)OR_NTOT);
         g[idx] = unit_of(&s);
         used += 2;
     }
@@ -380,8 +426,8 @@ typedef struct {
     or_slot *slots;
 } heap_t;
 
-/* src/screen.zig:166-192 */
-static void heap_offer(heap_t *h, const or_slot *s) {
+/* src/screen.zig:166-192; returns 1 when the slot entered the heap */
+static int heap_offer(heap_t *h, const or_slot *s) {
     if (h->n < h->k) {
         h->slots[h->n] = *s;
         uint64_t i = h->n;
@@ -394,9 +440,9 @@ static void heap_offer(heap_t *h, const or_slot *s) {
             h->slots[i] = t;
             i = parent;
         }
-        return;
+        return 1;
     }
-    if (s->score <= h->slots[0].score) return;
+    if (s->score <= h->slots[0].score) return 0;
     h->slots[0] = *s;
     uint64_t i = 0;
     for (;;) {
@@ -409,6 +455,7 @@ static void heap_offer(heap_t *h, const or_slot *s) {
         h->slots[i] = t;
         i = m;
     }
+    return 1;
 }
 
 /* -------------------------------------------------------------- worker --- */
@@ -421,31 +468,39 @@ typedef struct {
     uint64_t my_lambda;
     uint64_t seed;
     uint32_t stream;
+    int simd; /* score with or_score_simd */
     heap_t heap;
     uint64_t screens;
+    /* chunked form: candidates [i0, i0 + my_lambda) of the stream, starting
+     * at draw counter start0; every slot the (chunk-local) heap takes is
+     * also appended to emit */
+    uint64_t i0, start0;
+    or_slot *emit;
+    uint64_t n_emit, cap_emit;
 } job_t;
 
 /* src/screen.zig:225-255; the draw counter is tracked so that each kept slot
  * records where its candidate started. */
 static void run_job(job_t *j) {
-    uint64_t s = j->seed;
+    uint64_t s = j->seed + j->start0 * GAMMA;
     const int ne = j->ne;
     const float sigma = j->cfg->sigma, jump_p = j->cfg->jump_p, imm_p = j->cfg->imm_p;
     or_slot slot;
-    uint64_t start = 0;
-    for (uint64_t i = 0; i < j->my_lambda; i++) {
-        const or_genome *ea = &j->elites[(or_splitmix_next(&s) >> 32) % (uint64_t)ne];
-        const or_genome *eb = &j->elites[(or_splitmix_next(&s) >> 32) % (uint64_t)ne];
+    uint64_t start = j->start0;
+    for (uint64_t i = j->i0; i < j->i0 + j->my_lambda; i++) {
+        const or_genome *ea = &j->elites[(sm_next(&s) >> 32) % (uint64_t)ne];
+        const or_genome *eb = &j->elites[(sm_next(&s) >> 32) % (uint64_t)ne];
         float *g = slot.g;
         uint64_t used = 2;
         if (unit_of(&s) < imm_p) {
             for (int k = 0; k < OR_NTOT; k++) g[k] = unit_of(&s);
             used += 1 + OR_NTOT;
         } else {
+            /* bit 0 -> parent a, bit 1 -> parent b (a select, not a branch) */
+            const or_genome *par[2] = {ea, eb};
             for (int k = 0; k < OR_NTOT; k++) {
-                float pa = k < OR_NG ? ea->g[k] : ea->reg[k - OR_NG];
-                float pb = k < OR_NG ? eb->g[k] : eb->reg[k - OR_NG];
-                float v = (or_splitmix_next(&s) & 1) == 0 ? pa : pb;
+                const or_genome *e = par[sm_next(&s) & 1];
+                float v = k < OR_NG ? e->g[k] : e->reg[k - OR_NG];
                 float u = unit_of(&s);
                 v = v + ((u - 0.5f) * 2.0f) * sigma;
                 g[k] = clamp01(v);
@@ -454,11 +509,11 @@ static void run_job(job_t *j) {
         }
         used += 1;
         if (unit_of(&s) < jump_p) {
-            uint32_t idx = (uint32_t)((or_splitmix_next(&s) >> 32) % (uint64_t)OR_NTOT);
+            uint32_t idx = (uint32_t)((sm_next(&s) >> 32) % (uint64_t)OR_NTOT);
             g[idx] = unit_of(&s);
             used += 2;
         }
-        slot.score = or_score(j->m, g);
+        slot.score = j->simd ? score_simd(j->m, g) : score_scalar(j->m, g);
         slot.pa = ea->id;
         slot.pb = eb->id;
         slot.tribe = ea->tribe;
@@ -467,7 +522,13 @@ static void run_job(job_t *j) {
         slot.start = start;
         start += used;
         j->screens++;
-        heap_offer(&j->heap, &slot);
+        if (heap_offer(&j->heap, &slot) && j->cap_emit) {
+            if (j->n_emit == j->cap_emit) {
+                j->cap_emit *= 2;
+                j->emit = realloc(j->emit, j->cap_emit * sizeof(or_slot));
+            }
+            j->emit[j->n_emit++] = slot;
+        }
     }
 }
 
@@ -508,8 +569,8 @@ static void merge_desc(or_slot *a, or_slot *tmp, uint64_t n) {
 
 /* src/screen.zig:257-320.  `workers` OS threads execute the T stream jobs;
  * the result depends only on T (cfg->threads), as in the reference. */
-int or_screen(const or_model *m, const or_genome *elites, int ne, const or_cfg *cfg, int workers,
-              or_slot *out, uint64_t cap, uint64_t *n_out, uint64_t *screens) {
+static int screen_jobs(const or_model *m, const or_genome *elites, int ne, const or_cfg *cfg, int workers,
+                       int simd, or_slot *out, uint64_t cap, uint64_t *n_out, uint64_t *screens) {
     if (ne < 2) return OR_E_NEEDTWO;
     if (cfg->keep == 0) return OR_E_KEEP0;
     uint64_t T = cfg->threads < cfg->lambda ? cfg->threads : cfg->lambda;
@@ -526,6 +587,7 @@ int or_screen(const or_model *m, const or_genome *elites, int ne, const or_cfg *
         jobs[t].my_lambda = my;
         jobs[t].seed = or_stream_seed(cfg->seed, t);
         jobs[t].stream = (uint32_t)t;
+        jobs[t].simd = simd;
         jobs[t].heap.k = cfg->keep;
         jobs[t].heap.n = 0;
         jobs[t].heap.slots = malloc(cfg->keep * sizeof(or_slot));
@@ -553,6 +615,125 @@ int or_screen(const or_model *m, const or_genome *elites, int ne, const or_cfg *
         free(jobs[t].heap.slots);
     }
     free(jobs);
+    merge_desc(all, tmp, n);
+    uint64_t out_n = n < cfg->keep ? n : cfg->keep;
+    int rc = OR_OK;
+    if (out_n > cap) {
+        rc = OR_E_CAP;
+        out_n = cap;
+    }
+    memcpy(out, all, out_n * sizeof(or_slot));
+    free(all);
+    free(tmp);
+    *n_out = out_n;
+    *screens = total;
+    return rc;
+}
+
+int or_screen(const or_model *m, const or_genome *elites, int ne, const or_cfg *cfg, int workers,
+              or_slot *out, uint64_t cap, uint64_t *n_out, uint64_t *screens) {
+    return screen_jobs(m, elites, ne, cfg, workers, 0, out, cap, n_out, screens);
+}
+
+/* or_screen with the 16-wide score form (the reference's SIMD structure). */
+int or_screen_simd(const or_model *m, const or_genome *elites, int ne, const or_cfg *cfg, int workers,
+                   or_slot *out, uint64_t cap, uint64_t *n_out, uint64_t *screens) {
+    return screen_jobs(m, elites, ne, cfg, workers, 1, out, cap, n_out, screens);
+}
+
+/* Draw counter after `count` candidates that start at draw counter c: each
+ * candidate's length needs only its immigrant test (draw c+3) and its jump
+ * test (c+33 or c+62), read in the counter form out(n) = mix(seed + n g). */
+static uint64_t skip_candidates(uint64_t seed, uint64_t c, uint64_t count, float imm_p, float jump_p) {
+    for (uint64_t i = 0; i < count; i++) {
+        uint64_t st = seed + (c + 2) * GAMMA;
+        const int imm = unit_of(&st) < imm_p;
+        st = seed + (c + (imm ? 32 : 61)) * GAMMA;
+        const int jump = unit_of(&st) < jump_p;
+        c += (imm ? 33 : 62) + (jump ? 2 : 0);
+    }
+    return c;
+}
+
+/* or_screen in chunks of every stream, for many host threads (the survivors
+ * are identical).  Stream t's candidates are cut into `chunks` consecutive
+ * chunks; each chunk runs the worker loop with a heap of its own (same
+ * capacity and rule) and keeps every slot that heap took.  A candidate that
+ * heap rejected has `keep` earlier candidates of its stream scoring at least
+ * as much, so the stream's real heap rejects it as well; replaying the kept
+ * slots of the chunks in stream order through the real heap therefore gives
+ * the real heap exactly, ties and slot order included. */
+int or_screen_chunked(const or_model *m, const or_genome *elites, int ne, const or_cfg *cfg, int workers,
+                      int chunks, or_slot *out, uint64_t cap, uint64_t *n_out, uint64_t *screens) {
+    if (ne < 2) return OR_E_NEEDTWO;
+    if (cfg->keep == 0) return OR_E_KEEP0;
+    if (chunks < 1) chunks = 1;
+    uint64_t T = cfg->threads < cfg->lambda ? cfg->threads : cfg->lambda;
+    if (T < 1) T = 1;
+    const uint64_t per = cfg->lambda / T;
+    const uint64_t nj = T * (uint64_t)chunks;
+    job_t *jobs = calloc(nj, sizeof(job_t));
+    uint64_t issued = 0;
+    for (uint64_t t = 0; t < T; t++) {
+        const uint64_t my = (t == T - 1) ? cfg->lambda - issued : per;
+        issued += per;
+        const uint64_t seed = or_stream_seed(cfg->seed, t);
+        uint64_t i0 = 0, c = 0;
+        for (int q = 0; q < chunks; q++) {
+            job_t *j = &jobs[t * (uint64_t)chunks + (uint64_t)q];
+            const uint64_t n = my / (uint64_t)chunks + ((uint64_t)q < my % (uint64_t)chunks ? 1 : 0);
+            j->m = m;
+            j->elites = elites;
+            j->ne = ne;
+            j->cfg = cfg;
+            j->my_lambda = n;
+            j->seed = seed;
+            j->stream = (uint32_t)t;
+            j->simd = 1;
+            j->heap.k = cfg->keep;
+            j->heap.slots = malloc(cfg->keep * sizeof(or_slot));
+            j->i0 = i0;
+            j->start0 = c;
+            j->cap_emit = 1024;
+            j->emit = malloc(j->cap_emit * sizeof(or_slot));
+            i0 += n;
+            c = skip_candidates(seed, c, n, cfg->imm_p, cfg->jump_p);
+        }
+    }
+    if (workers < 1) workers = 1;
+    if ((uint64_t)workers > nj) workers = (int)nj;
+    pool_t pool = {jobs, nj, 0, PTHREAD_MUTEX_INITIALIZER};
+    pthread_t *th = calloc((size_t)workers, sizeof(pthread_t));
+    for (int w = 1; w < workers; w++) pthread_create(&th[w], NULL, pool_main, &pool);
+    pool_main(&pool);
+    for (int w = 1; w < workers; w++) pthread_join(th[w], NULL);
+    free(th);
+
+    /* replay per stream, then the merge of or_screen */
+    uint64_t total = 0, n = 0;
+    heap_t *real = calloc(T, sizeof(heap_t));
+    for (uint64_t t = 0; t < T; t++) {
+        real[t].k = cfg->keep;
+        real[t].slots = malloc(cfg->keep * sizeof(or_slot));
+        for (int q = 0; q < chunks; q++) {
+            job_t *j = &jobs[t * (uint64_t)chunks + (uint64_t)q];
+            total += j->screens;
+            for (uint64_t e = 0; e < j->n_emit; e++) heap_offer(&real[t], &j->emit[e]);
+            free(j->emit);
+            free(j->heap.slots);
+        }
+        n += real[t].n;
+    }
+    free(jobs);
+    or_slot *all = malloc((n ? n : 1) * sizeof(or_slot));
+    or_slot *tmp = malloc((n ? n : 1) * sizeof(or_slot));
+    uint64_t k = 0;
+    for (uint64_t t = 0; t < T; t++) {
+        memcpy(all + k, real[t].slots, real[t].n * sizeof(or_slot));
+        k += real[t].n;
+        free(real[t].slots);
+    }
+    free(real);
     merge_desc(all, tmp, n);
     uint64_t out_n = n < cfg->keep ? n : cfg->keep;
     int rc = OR_OK;

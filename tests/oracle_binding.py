@@ -56,11 +56,18 @@ class Oracle:
         L.or_load_elites.argtypes = [C.c_char_p, C.POINTER(OrGenome), C.c_int, C.POINTER(C.c_int)]
         L.or_score.restype = C.c_float
         L.or_score.argtypes = [C.POINTER(OrModel), C.POINTER(C.c_float)]
+        L.or_score_simd.restype = C.c_float
+        L.or_score_simd.argtypes = [C.POINTER(OrModel), C.POINTER(C.c_float)]
+        L.or_clamp01.restype = C.c_float
+        L.or_clamp01.argtypes = [C.c_float]
         L.or_synth.argtypes = [C.POINTER(OrGenome), C.c_int, C.c_uint64, C.c_uint64, C.c_float, C.c_float,
                                C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_uint32),
                                C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.or_screen.argtypes = [C.POINTER(OrModel), C.POINTER(OrGenome), C.c_int, C.POINTER(OrCfg), C.c_int,
                                 C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.or_screen_simd.argtypes = L.or_screen.argtypes
+        L.or_screen_chunked.argtypes = [C.POINTER(OrModel), C.POINTER(OrGenome), C.c_int, C.POINTER(OrCfg), C.c_int,
+                                        C.c_int, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.or_fmt_d4.argtypes = [C.c_float, C.c_char_p, C.c_size_t]
         L.or_write_promoted.argtypes = [C.c_char_p, C.c_void_p, C.c_uint64]
         L.or_write_count.argtypes = [C.c_char_p, C.c_uint64]
@@ -107,6 +114,10 @@ class Oracle:
         arr = np.ascontiguousarray(np.asarray(g, dtype=np.float32))
         return self.L.or_score(C.byref(m), arr.ctypes.data_as(C.POINTER(C.c_float)))
 
+    def score_simd(self, m: OrModel, g) -> float:
+        arr = np.ascontiguousarray(np.asarray(g, dtype=np.float32))
+        return self.L.or_score_simd(C.byref(m), arr.ctypes.data_as(C.POINTER(C.c_float)))
+
     def synth(self, elites, stream_seed: int, start: int, sigma: float, jump_p: float, imm_p: float):
         g = (C.c_float * NTOT)()
         L = C.c_uint32()
@@ -117,17 +128,36 @@ class Oracle:
         return np.array(g[:], dtype=np.float32), L.value, ia.value, ib.value
 
     def screen(self, m: OrModel, elites, lam: int, keep: int, seed: int, threads: int = 4, sigma: float = 0.12,
-               jump_p: float = 0.02, imm_p: float = 0.01, workers: int = 0):
+               jump_p: float = 0.02, imm_p: float = 0.01, workers: int = 0, simd: bool = False):
+        """or_screen, or with simd=True or_screen_simd (the 16-wide score form;
+        identical results, the form the CPU baseline times)."""
         cfg = OrCfg(lam, keep, seed, threads, sigma, jump_p, imm_p, 0.0)
         cap = max(1, keep)
         out = np.zeros(cap, dtype=SLOT_DTYPE)
         n = C.c_uint64()
         scr = C.c_uint64()
         w = workers or min(os.cpu_count() or 1, 16)
-        rc = self.L.or_screen(C.byref(m), elites, len(elites), C.byref(cfg), w, out.ctypes.data, cap,
-                              C.byref(n), C.byref(scr))
+        fn = self.L.or_screen_simd if simd else self.L.or_screen
+        rc = fn(C.byref(m), elites, len(elites), C.byref(cfg), w, out.ctypes.data, cap, C.byref(n), C.byref(scr))
         if rc != 0:
             raise ValueError(f"or_screen rc={rc}")
+        return out[: n.value].copy(), scr.value
+
+    def screen_chunked(self, m: OrModel, elites, lam: int, keep: int, seed: int, threads: int = 4,
+                       sigma: float = 0.12, jump_p: float = 0.02, imm_p: float = 0.01, workers: int = 0,
+                       chunks: int = 64):
+        """or_screen_chunked: the same survivors as screen(), each stream cut
+        into `chunks` pieces run on up to `workers` host threads."""
+        cfg = OrCfg(lam, keep, seed, threads, sigma, jump_p, imm_p, 0.0)
+        cap = max(1, keep)
+        out = np.zeros(cap, dtype=SLOT_DTYPE)
+        n = C.c_uint64()
+        scr = C.c_uint64()
+        w = workers or min(os.cpu_count() or 1, 16)
+        rc = self.L.or_screen_chunked(C.byref(m), elites, len(elites), C.byref(cfg), w, chunks, out.ctypes.data, cap,
+                                      C.byref(n), C.byref(scr))
+        if rc != 0:
+            raise ValueError(f"or_screen_chunked rc={rc}")
         return out[: n.value].copy(), scr.value
 
     def fmt(self, v: float) -> str:

@@ -161,3 +161,49 @@ def test_caller_cfg_mirrors_evolve_call():
     assert cfg.jump_p == float(np.float32(0.02)) and cfg.imm_p == float(np.float32(0.01))
     assert scr.caller_cfg(10, 12, 1, 0, 0).keep == 12 - 2 - 2
     assert scr.caller_cfg(10, 12, 1, 0, 0, keep=5).keep == 5
+
+
+def _evolve_promoted(path, ng=29):
+    """brain/evolve.py:445-454 restated: lines of 5 + NG tokens, pa/pb/tribe
+    as ints and the genes as floats; every other line is dropped."""
+    out = []
+    for line in open(path):
+        t = line.split()
+        if len(t) != 5 + ng:
+            continue
+        out.append({"pa": int(t[2]), "pb": int(t[3]), "tribe": int(t[4]), "g": [float(x) for x in t[5:]]})
+    return out
+
+
+def test_v6_writer_lines_are_kept_by_evolve(tmp_path):
+    rng = np.random.default_rng(5)
+    s = np.zeros(7, dtype=scr.SURVIVOR_DTYPE)
+    s["score"] = rng.standard_normal(7)
+    s["g"] = rng.random((7, 29)).astype(np.float32)
+    s["g"][0, :3] = [0.0, 1.0, 5e-5]
+    s["pa"], s["pb"], s["tribe"] = rng.integers(0, 10**12, 7), rng.integers(0, 10**12, 7), rng.integers(0, 12, 7)
+    zig, v6 = str(tmp_path / "zig.txt"), str(tmp_path / "v6.txt")
+    scr.write_promoted(zig, s)
+    scr.write_promoted_v6(v6, s)
+    assert _evolve_promoted(zig) == []  # R2: the reference's 29-token lines are all dropped
+    kept = _evolve_promoted(v6)
+    assert len(kept) == 7
+    for r, row in zip(s, kept):
+        assert (row["pa"], row["pb"], row["tribe"]) == (r["pa"], r["pb"], r["tribe"])
+        assert row["g"] == [float(scr.format_gene(float(x))) for x in r["g"]]
+    # the v6 line's first 29 tokens are the reference line, token for token
+    for a, b in zip(open(zig), open(v6)):
+        assert b.split()[:29] == a.split()
+    # and the next generation reads them back through the v6 parse (R1: genes 16..21 read as 0)
+    for line, r in zip(open(v6), s):
+        g = scr.parse_line(line.rstrip("\n"))
+        assert g is not None and g.pa == r["pa"] and list(g.g[16:22]) == [0.0] * 6
+        assert np.float32(g.g[22]) == np.float32(float(scr.format_gene(float(r["g"][22]))))
+
+
+def test_v6_format_environment_switch_is_checked(tmp_path, golden, monkeypatch, capsys):
+    monkeypatch.setenv("CSFUNNEL_PROMOTED_FORMAT", "bogus")
+    rc = scr.screen_cmd(["--model", os.path.join(golden, "c0_model.txt"), "--elites",
+                         os.path.join(golden, "c0_elites.txt"), "--out", str(tmp_path / "p.txt")])
+    assert rc == 1 and "CSFUNNEL_PROMOTED_FORMAT" in capsys.readouterr().err
+    assert not (tmp_path / "p.txt").exists()

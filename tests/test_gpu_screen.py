@@ -144,6 +144,24 @@ def test_screen_cmd_files_match_oracle(oracle, golden, tmp_path):
     assert open(cnt).read() == f"{total}\n" == "300000\n"
 
 
+@pytest.mark.parametrize("threads", [4, 1])
+def test_record_overflow_split_and_retry(dev, oracle, golden, threads):
+    # Record buffers of 2048 records (one block emits at most 1987): the first
+    # launches emit every candidate of 1 and 3 blocks per stream while the
+    # heaps fill, so they overflow and are split -- by streams while more than
+    # one stream is active, then by blocks inside one stream -- and redone
+    # with the other record buffer's launch still queued behind them.
+    m, e = _c0(golden)
+    dev.set_record_cap(2048)
+    try:
+        _, st = _run_both(dev, oracle, m, e, 1 << 20, 256, 41, threads, 0.12, 0.01, 0.01)
+    finally:
+        dev.set_record_cap(1 << 20)
+    assert st["retries"] >= (6 if threads == 4 else 2), st
+    with pytest.raises(CsError):
+        dev.set_record_cap(1024)
+
+
 def test_keep_zero_and_empty(dev, golden):
     m, e = _c0(golden)
     with pytest.raises(CsError):
@@ -215,3 +233,108 @@ def test_slow_mutation_path_edge_inputs(dev, oracle, golden, tmp_path):
     pop.write_text("\n".join([" ".join(toks)] + lines[1:]) + "\n")
     got, _ = _run_both(dev, oracle, m, str(pop), 100000, 64, 32, 4)
     assert np.all((got["g"] >= 0) & (got["g"] <= 1))
+
+
+def test_arena_executable_as_evolve_calls_it(oracle, golden, tmp_path):
+    # brain/evolve.py:421-438 restated: the elites file it writes (genome_line,
+    # 34-token v6 lines of the top E), then `<zig> screen ...` as a subprocess
+    # with exactly its argv; both output files must equal the oracle writer's.
+    import subprocess
+    import sys
+    pop, seed, shard, gen, screens, threads = 96, 7, 3, 2, 300_000, 4
+    E = max(2, pop // 8)
+    K = pop - E - max(2, pop // 8)
+    lines = open(os.path.join(golden, "seed_pop.txt")).read().splitlines()[:E]
+    elites = tmp_path / "elites.txt"
+    elites.write_text("".join(line + "\n" for line in lines))
+    model = os.path.join(golden, "surrogate_161.txt")
+    prom, cnt = tmp_path / "promoted.txt", tmp_path / "screens.txt"
+    sigma = f"{0.12 * (0.93 ** gen):.4f}"
+    argv = ["screen", "--model", model, "--elites", str(elites), "--out", str(prom), "--count-out", str(cnt),
+            "--lambda", str(screens), "--keep", str(K), "--seed", str(seed * 1000003 + shard * 131 + gen),
+            "--threads", str(threads), "--sigma", sigma, "--jump", "0.02", "--imm", "0.01"]
+    exe = os.path.join(os.path.dirname(golden), "..", "codec-arena_amd", "bin", "arena")
+    print(f"case arena subprocess K={K} sigma={sigma}", flush=True)
+    env = dict(os.environ)
+    env.pop("CSFUNNEL_PROMOTED_FORMAT", None)
+    r = subprocess.run([sys.executable, exe] + argv, capture_output=True, text=True, env=env, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert r.stderr.strip().endswith(f"screen: {screens} candidates scored, top {K} promoted")
+    cfg = scr.caller_cfg(screens, pop, seed, shard, gen, threads)
+    assert (cfg.keep, cfg.seed) == (K, seed * 1000003 + shard * 131 + gen)
+    ref, total = oracle.screen(oracle.load_model(model), oracle.load_elites(str(elites)), cfg.lambda_, cfg.keep,
+                               cfg.seed, threads, cfg.sigma, cfg.jump_p, cfg.imm_p)
+    oracle.write_promoted(str(tmp_path / "ref.txt"), ref)
+    assert prom.read_bytes() == (tmp_path / "ref.txt").read_bytes()
+    assert cnt.read_text() == f"{total}\n" == f"{screens}\n"
+    # the named v6 choice: evolve.py keeps these lines (all 29 genes)
+    env["CSFUNNEL_PROMOTED_FORMAT"] = "v6"
+    r = subprocess.run([sys.executable, exe] + argv + ["--v6-out", str(tmp_path / "v6b.txt")], capture_output=True,
+                       text=True, env=env, timeout=300)
+    assert r.returncode == 0, r.stderr
+    v6 = prom.read_text().splitlines()
+    assert v6 == (tmp_path / "v6b.txt").read_text().splitlines()
+    assert len(v6) == K and all(len(line.split()) == 34 for line in v6)
+    for line, zline, s in zip(v6, open(tmp_path / "ref.txt"), ref):
+        assert line.split()[:29] == zline.split()
+        assert [float(x) for x in line.split()[5:]] == [float(oracle.fmt(float(x))) for x in s["g"]]
+
+
+def _chain_elites(survivors, gen):
+    """Survivors of generation gen as the next generation's elites (ids
+    128 gen + rank, genes 0..22 and regulators from the survivor's 29 genes),
+    for the device and for the oracle."""
+    from csfunnel import _abi
+    from oracle_binding import OrGenome
+    n = len(survivors)
+    dev_el, or_el = (_abi.Genome * n)(), (OrGenome * n)()
+    for r, s in enumerate(survivors):
+        for e in (dev_el[r], or_el[r]):
+            e.id, e.gen = gen * 128 + r, gen + 1
+            e.pa, e.pb, e.tribe = int(s["pa"]), int(s["pb"]), int(s["tribe"])
+            for k in range(23):
+                e.g[k] = float(s["g"][k])
+            for k in range(6):
+                e.reg[k] = float(s["g"][23 + k])
+    return dev_el, or_el
+
+
+@pytest.mark.timeout(1200)
+def test_production_chained_generation_bit_exact(oracle, golden):
+    # The production shard at full size: generation 0 (2.65e9 candidates,
+    # 128 elites, keep 128, 4 streams) on the GPU, its survivors become the
+    # elites of generation 1 (seed 1, sigma "%.4f" % (0.12 * 0.93)), and
+    # generation 1 on the GPU must equal the oracle's -- every survivor field
+    # and screens_count -- over all 2.65e9 candidates.
+    m, e = _c0(golden, "c1_elites.txt")
+    model, mo = scr.load_model(m), oracle.load_model(m)
+    lam = 2_650_000_000
+    with scr.Screen(0) as d:
+        cfg0 = scr.Cfg(lam, 128, 0, 4, float(np.float32(0.12)), float(np.float32(0.01)), float(np.float32(0.01)))
+        print("case chained production generation 0 (GPU)", flush=True)
+        s0, _ = d.run(model, scr.load_elites(e), cfg0)
+        el1, or_el1 = _chain_elites(s0, 0)
+        cfg1 = scr.Cfg(lam, 128, 1, 4, scr.parse_f32(f"{0.12 * 0.93:.4f}"), cfg0.jump_p, cfg0.imm_p)
+        print("case chained production generation 1 (GPU)", flush=True)
+        got, screens = d.run(model, el1, cfg1)
+        assert d.stats()["candidates"] == lam
+    print("case chained production generation 1 (oracle, 4 streams in 64 chunks each on 16 host threads)", flush=True)
+    ref, rscreens = oracle.screen_chunked(mo, or_el1, lam, 128, 1, 4, cfg1.sigma, cfg1.jump_p, cfg1.imm_p,
+                                          workers=16, chunks=64)
+    assert screens == rscreens == lam
+    _same(got, ref)
+
+
+@pytest.mark.parametrize("sigma", [0.0, 0.12, 1e-40])
+def test_negative_zero_parent_genes(dev, oracle, golden, tmp_path, sigma):
+    # parents read as -0.0: with sigma 0 half of the mutated genes are
+    # -0 + -0 = -0 before the clamp, which gives +0 (clamp01(-0) = +0)
+    rng = np.random.default_rng(3)
+    pop = tmp_path / "negzero.txt"
+    with open(pop, "w") as f:
+        for i in range(16):
+            genes = ["-0.0000" if rng.random() < 0.5 else f"{x:.4f}" for x in rng.random(16)]
+            f.write(f"{i} 0 0 0 {i % 12} " + " ".join(genes) + "\n")
+    m, _ = _c0(golden)
+    got, _ = _run_both(dev, oracle, m, str(pop), 200000, 128, 8, 4, sigma=sigma, jump_p=0.0, imm_p=0.0)
+    assert not np.any(np.signbit(got["g"]))

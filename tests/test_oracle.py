@@ -254,6 +254,194 @@ def test_mutation_bounded_by_sigma(oracle, golden):
         start += L
 
 
+def _flat_pop(tmp_path, ne=8, value="0.5000"):
+    """ne 33-token (v5) lines with every gene and regulator = value: genes
+    0..21 and the regulators parse to it, gene 22 to 0 (src/genome.zig:116-188)."""
+    p = tmp_path / f"flat_{ne}.txt"
+    p.write_text("".join(f"{i} 0 0 0 {i % 12} " + " ".join([value] * 28) + "\n" for i in range(ne)))
+    return str(p)
+
+
+def _draws(oracle, seed, c, n):
+    """out(c + 1) .. out(c + n) of the stream with state seed (splitmix64)."""
+    return oracle.splitmix((seed + c * GAMMA) % 2**64, n)
+
+
+PARENT_HALF = [k for k in range(29) if k != 22]  # genes whose parents are 0.5 in _flat_pop
+
+
+def test_mutation_closed_form_at_sigma_half(oracle, tmp_path):
+    # parent 0.5, sigma 0.5: 0.5 + ((u - 0.5) * 2) * 0.5 = u exactly, so every
+    # such gene is the mutation unit itself, on the 2^-24 lattice, and the
+    # offsets g - 0.5 fill [-sigma, sigma).  A dropped "* 2" halves the range,
+    # a dropped "- 0.5" pushes every gene to >= 0.5 (and clamps half of them).
+    el = oracle.load_elites(_flat_pop(tmp_path))
+    seed, start, offs = 12345, 0, []
+    for _ in range(3000):
+        g, L, ia, ib = oracle.synth(el, seed, start, 0.5, 0.0, 0.0)
+        assert L == 62
+        d = _draws(oracle, seed, start, L)
+        u = (d[4:62:2] >> np.uint64(40)).astype(np.float64) * 2.0**-24  # draws c+5, c+7, ..., c+61
+        assert np.array_equal(g[PARENT_HALF].astype(np.float64), u[PARENT_HALF])
+        offs.append(g[PARENT_HALF].astype(np.float64) - 0.5)
+        start += L
+    offs = np.concatenate(offs)
+    assert offs.min() >= -0.5 and offs.max() < 0.5
+    assert offs.min() <= -0.5 + 1e-3 * 0.5 and offs.max() >= 0.5 - 1e-3 * 0.5
+    assert abs(offs.mean()) < 0.01
+
+
+@pytest.mark.parametrize("k", [3, 7, 12])
+def test_mutation_range_at_power_of_two_sigma(oracle, tmp_path, k):
+    # sigma = 2^-k: offsets stay inside [-sigma, sigma] (one rounding of the
+    # sum at most) and reach within 1e-3 sigma of both ends
+    el = oracle.load_elites(_flat_pop(tmp_path))
+    sigma, start, offs = 2.0**-k, 0, []
+    for _ in range(4000):
+        g, L, _, _ = oracle.synth(el, 777 + k, start, sigma, 0.0, 0.0)
+        offs.append(g[PARENT_HALF].astype(np.float64) - 0.5)
+        start += L
+    offs = np.concatenate(offs)
+    ulp = 2.0**-25
+    assert offs.min() >= -sigma - ulp and offs.max() <= sigma + ulp
+    assert offs.min() <= -sigma + 1e-3 * sigma and offs.max() >= sigma - 1e-3 * sigma
+
+
+@pytest.mark.parametrize("ne", [128, 96, 7])
+def test_elite_indices_are_high_words_mod_ne(oracle, tmp_path, ne):
+    # ea = elites[(out(c+1) >> 32) % ne], eb = elites[(out(c+2) >> 32) % ne]
+    el = oracle.load_elites(_flat_pop(tmp_path, ne))
+    seed, start = 0xDEADBEEF, 0
+    for _ in range(2000):
+        _, L, ia, ib = oracle.synth(el, seed, start, 0.1, 0.02, 0.01)
+        d = _draws(oracle, seed, start, 2)
+        assert (ia, ib) == (int(d[0] >> np.uint64(32)) % ne, int(d[1] >> np.uint64(32)) % ne)
+        start += L
+
+
+def test_immigrant_and_jump_tests_read_their_draws(oracle, tmp_path):
+    # immigrant iff unit(out(c+3)) < imm_p; the jump test is out(c+33) for an
+    # immigrant and out(c+62) otherwise, with strict "<"
+    el = oracle.load_elites(_flat_pop(tmp_path))
+    seed, start = 4242, 0
+    kinds = set()
+    for _ in range(3000):
+        _, L, _, _ = oracle.synth(el, seed, start, 0.1, 0.5, 0.5)
+        d = _draws(oracle, seed, start, 64)
+        unit = lambda x: float(int(x) >> 40) * 2.0**-24  # noqa: E731
+        imm = unit(d[2]) < 0.5
+        jump = unit(d[32 if imm else 61]) < 0.5
+        assert L == (33 if imm else 62) + (2 if jump else 0)
+        kinds.add(L)
+        start += L
+    assert kinds == {33, 35, 62, 64}
+
+
+def test_event_tests_are_strict_at_the_rate(oracle, tmp_path):
+    # a rate equal to the drawn unit is not an event (unit < p), the next f32
+    # above it is: immigrant test at out(c+3), jump test at out(c+62)
+    el = oracle.load_elites(_flat_pop(tmp_path))
+    unit = lambda x: np.float32(float(int(x) >> 40) * 2.0**-24)  # noqa: E731
+    for seed, c in ((1, 0), (2, 1000), (3, 77)):
+        d = _draws(oracle, seed, c, 64)
+        ui, uj = unit(d[2]), unit(d[61])
+        above = lambda u: float(np.nextafter(u, np.float32(2)))  # noqa: E731
+        assert oracle.synth(el, seed, c, 0.1, 0.0, float(ui))[1] == 62
+        assert oracle.synth(el, seed, c, 0.1, 0.0, above(ui))[1] == 33
+        assert oracle.synth(el, seed, c, 0.1, float(uj), 0.0)[1] == 62
+        assert oracle.synth(el, seed, c, 0.1, above(uj), 0.0)[1] == 64
+
+
+def test_jump_index_uniform_over_29_and_value_on_lattice(oracle, tmp_path):
+    # sigma 0 and every parent gene 0.5 except gene 22 (0): the jumped gene is
+    # the one that moved; its index is (out(c+63) >> 32) % 29, uniform over
+    # the 29 genes (chi-square, 28 degrees of freedom), and its value is
+    # unit(out(c+64)) on the 2^-24 lattice
+    el = oracle.load_elites(_flat_pop(tmp_path))
+    base = np.full(29, 0.5, np.float32)
+    base[22] = 0.0
+    seed, start = 99, 0
+    counts = np.zeros(29, int)
+    for _ in range(29 * 400):
+        g, L, _, _ = oracle.synth(el, seed, start, 0.0, 1.0, 0.0)
+        assert L == 64
+        d = _draws(oracle, seed, start, 64)
+        idx = int(d[62] >> np.uint64(32)) % 29
+        val = float(int(d[63]) >> 40) * 2.0**-24
+        expect = base.copy()
+        expect[idx] = val
+        assert np.array_equal(g, expect)
+        assert float(g[idx]) * 2**24 == int(float(g[idx]) * 2**24)
+        counts[idx] += 1
+        start += L
+    chi2 = float(((counts - 400.0) ** 2 / 400.0).sum())
+    assert chi2 < 80.0, (chi2, counts)  # p < 1e-6 for 28 degrees of freedom
+
+
+def test_clamp01_special_values(oracle):
+    # NaN -> 1 and -0 -> +0 (the constants win ties and unordered compares)
+    cases = [(float("nan"), 1.0), (-0.0, 0.0), (2.0, 1.0), (-1.0, 0.0), (float("inf"), 1.0),
+             (float("-inf"), 0.0), (0.5, 0.5), (1e-45, 1e-45), (1.0, 1.0), (0.0, 0.0)]
+    for x, want in cases:
+        got = np.float32(oracle.L.or_clamp01(np.float32(x)))
+        assert got.view(np.uint32) == np.float32(want).view(np.uint32), (x, got)
+
+
+def test_simd_score_form_equals_scalar(oracle, golden):
+    rng = np.random.default_rng(17)
+    for n in (153, 159, 161, 164, 165, 166):
+        m = oracle.model_from_compact((rng.standard_normal(n) / np.sqrt(n)).astype(np.float32))
+        for _ in range(500):
+            g = rng.random(29).astype(np.float32)
+            g[rng.random(29) < 0.1] = 0.0
+            a = np.float32(oracle.score(m, g))
+            b = np.float32(oracle.score_simd(m, g))
+            assert a.view(np.uint32) == b.view(np.uint32)
+
+
+@pytest.mark.parametrize("lam,T,sigma,jump,imm", [(1 << 20, 4, 0.12, 0.01, 0.01), (1 << 18, 1, 0.12, 0.01, 0.01),
+                                                 (200000, 3, 0.5, 0.5, 0.5), (100000, 4, 0.0, 0.0, 0.0)])
+def test_simd_screen_equals_scalar_screen(oracle, golden, lam, T, sigma, jump, imm):
+    m = oracle.load_model(os.path.join(golden, "c0_model.txt"))
+    el = _elites(oracle, golden)
+    args = (m, el, lam, 256, 5, T, float(np.float32(sigma)), float(np.float32(jump)), float(np.float32(imm)))
+    a, sa = oracle.screen(*args)
+    b, sb = oracle.screen(*args, simd=True)
+    assert sa == sb == lam
+    assert a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("case", ["c0", "c0_T1", "rates", "zero_model", "coarse", "keep_big"])
+def test_chunked_screen_equals_sequential(oracle, golden, tmp_path, case):
+    # the chunked form (per-chunk heaps, replay of what they took) against the
+    # plain per-stream worker loop, including exact ties straddling the cut
+    m = oracle.load_model(os.path.join(golden, "c0_model.txt"))
+    el = _elites(oracle, golden)
+    f = lambda x: float(np.float32(x))  # noqa: E731
+    args = dict(lam=1 << 20, keep=256, seed=9, threads=4, sigma=f(0.12), jump_p=f(0.01), imm_p=f(0.01))
+    if case == "c0_T1":
+        args.update(threads=1, lam=1 << 19)
+    elif case == "rates":
+        args.update(lam=300000, threads=3, sigma=f(0.5), jump_p=f(0.5), imm_p=f(0.5))
+    elif case == "zero_model":
+        m = oracle.model_from_compact(np.zeros(153, np.float32))
+    elif case == "coarse":
+        rng = np.random.default_rng(8)
+        m = oracle.model_from_compact(rng.integers(-2, 3, 153).astype(np.float32))
+        pop = tmp_path / "coarse.txt"
+        pop.write_text("".join(f"{i} 0 0 0 {i} " + " ".join(str(x) for x in rng.integers(0, 2, 16)) + "\n"
+                               for i in range(8)))
+        el = oracle.load_elites(str(pop))
+        args.update(lam=400000, keep=500, sigma=0.0, jump_p=0.0, imm_p=0.0)
+    elif case == "keep_big":
+        args.update(lam=50000, keep=20000)
+    ref, rs = oracle.screen(m, el, **args)
+    for chunks in (1, 7, 64):
+        got, gs = oracle.screen_chunked(m, el, chunks=chunks, **args)
+        assert gs == rs == args["lam"]
+        assert got.tobytes() == ref.tobytes(), chunks
+
+
 def _enumerate(oracle, el, seed, lam, sigma, jump_p, imm_p):
     """Every candidate of one stream by chaining single-candidate synthesis."""
     out = []
