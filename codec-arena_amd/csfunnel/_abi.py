@@ -72,6 +72,9 @@ SIGNATURES = {
     "cs_screen": (C.c_int, [C.c_void_p, C.POINTER(Model), C.POINTER(Genome), C.c_int,
                             C.POINTER(ScreenCfg), C.POINTER(Survivor), C.c_uint64,
                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "cs_screen_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Model), C.POINTER(Genome), C.c_int,
+                                    C.POINTER(ScreenCfg), C.POINTER(Survivor), C.c_uint64,
+                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "cs_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ScreenStats)]),
     "cs_set_record_cap": (C.c_int, [C.c_void_p, C.c_uint32]),
     "cs_set_offset_mode": (C.c_int, [C.c_void_p, C.c_int]),

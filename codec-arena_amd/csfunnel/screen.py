@@ -154,8 +154,10 @@ class Screen:
         except Exception:
             pass
 
-    def run(self, model: Model, elites, cfg: Cfg) -> Tuple[np.ndarray, int]:
-        """Survivors (SURVIVOR_DTYPE, reference output order) and screens_count."""
+    def run(self, model: Model, elites, cfg: Cfg, noalloc: bool = False) -> Tuple[np.ndarray, int]:
+        """Survivors (SURVIVOR_DTYPE, reference output order) and screens_count.
+        noalloc: through cs_screen_noalloc (device buffers from the library's
+        static scratch array instead of hipMalloc)."""
         ne = len(elites)
         if isinstance(elites, C.Array):
             earr = elites
@@ -166,8 +168,9 @@ class Screen:
         n_out = C.c_uint64(0)
         screens = C.c_uint64(0)
         c = cfg.to_c()
-        check(lib().cs_screen(self._h, C.byref(model.raw), earr, ne, C.byref(c),
-                              out.ctypes.data_as(C.POINTER(Survivor)), cap, C.byref(n_out), C.byref(screens)))
+        fn = lib().cs_screen_noalloc if noalloc else lib().cs_screen
+        check(fn(self._h, C.byref(model.raw), earr, ne, C.byref(c), out.ctypes.data_as(C.POINTER(Survivor)), cap,
+                 C.byref(n_out), C.byref(screens)))
         return out[: n_out.value].copy(), screens.value
 
     def set_offset_mode(self, mode: int) -> None:

@@ -59,6 +59,7 @@ struct cs_ctx {
     bool force_maps = false;  // testing hook: always use the transfer maps
     int n_cu = 256;
     int n_mlp = 0;  // live cs_mlp handles bound to this context
+    void *scratch = nullptr;  // this device's copy of the static scratch array
 };
 
 // Learned surrogate state (mlp_device.hip): FP32 master parameters, AdamW

@@ -31,6 +31,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1011,6 +1012,37 @@ static double ms_between(hipEvent_t a, hipEvent_t b) {
 
 static float nan_f() { return std::nanf(""); }
 
+// ---------------------------------------------- allocation-free scratch
+
+// Device memory of cs_screen_noalloc: every buffer of such a call is carved
+// out of this array, which the code object brings with it, so the call never
+// allocates.  Each GPU holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (kept per device in scratch_address).  Every call on one GPU
+// shares the one array: calls that overlap on a GPU race.
+constexpr size_t SCRATCH_BYTES = (size_t)64 << 20;
+constexpr uint32_t NOALLOC_REC_CAP = 1u << 16;  // records per record buffer (2 x 9.96 MB)
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of this GPU's scratch array; `device` must be current.
+static int scratch_address(int device, void **out) {
+    if (device < 0 || device >= MAX_DEVICES) return fail(CS_E_NODEV, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(CS_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return CS_OK;
+}
+
 struct HostState {
     cs_ctx *ctx;
     ScreenBufs *B = nullptr;    // device buffers of this call
@@ -1379,9 +1411,11 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
     return CS_OK;
 }
 
+// noalloc: take every device buffer from the context device's scratch array
+// (no hipMalloc / hipFree), with record buffers of at most NOALLOC_REC_CAP.
 static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne,
                        const cs_screen_cfg *cfg, const cs_screen_opts *opts, cs_survivor *out, uint64_t cap,
-                       uint64_t *n_out, uint64_t *screens) {
+                       uint64_t *n_out, uint64_t *screens, bool noalloc = false) {
     auto wall0 = std::chrono::steady_clock::now();
     ctx->stats = cs_screen_stats{};
     if (ne < 2) return fail(CS_E_NEEDTWO, "NeedTwoElites");
@@ -1404,9 +1438,15 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     if (T64 > (1u << 24)) return fail(CS_E_BADARGS, "threads must be <= 2^24");
     const int T = (int)T64;
     HostState h;
+    ScreenBufs carved;  // noalloc: pieces of the scratch array
     h.ctx = ctx;
-    h.B = &ctx->sb;
-    h.rec_cap = ctx->rec_cap;
+    h.B = noalloc ? &carved : &ctx->sb;
+    if (noalloc) {
+        if (!ctx->scratch) return fail(CS_E_INTERNAL, "scratch array address unknown for this device");
+        h.arena = (char *)ctx->scratch;
+        h.arena_cap = SCRATCH_BYTES;
+    }
+    h.rec_cap = noalloc ? std::min<uint32_t>(ctx->rec_cap, NOALLOC_REC_CAP) : ctx->rec_cap;
     h.T = T;
     h.elites = elites;
     h.si.resize(T);
@@ -1618,6 +1658,10 @@ int cs_ctx_create(int device, cs_ctx **out) {
     cs_ctx *c = new cs_ctx();
     c->device = device;
     c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    if (int rc = scratch_address(device, &c->scratch)) {
+        delete c;
+        return rc;
+    }
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -1671,6 +1715,15 @@ int cs_screen_ex(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, in
     clear_error();
     HIPCHK(hipSetDevice(ctx->device));
     return screen_impl(ctx, model, elites, ne, cfg, opts, out, cap, n_out, screens);
+}
+
+int cs_screen_noalloc(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne,
+                      const cs_screen_cfg *cfg, cs_survivor *out, uint64_t cap, uint64_t *n_out, uint64_t *screens) {
+    if (!ctx || !model || !cfg || !n_out || !screens || (ne > 0 && !elites) || (cap > 0 && !out))
+        return fail(CS_E_BADARGS, "null argument");
+    clear_error();
+    HIPCHK(hipSetDevice(ctx->device));
+    return screen_impl(ctx, model, elites, ne, cfg, nullptr, out, cap, n_out, screens, true);
 }
 
 int cs_set_record_cap(cs_ctx *ctx, uint32_t cap) {

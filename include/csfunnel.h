@@ -138,6 +138,23 @@ int cs_screen(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int n
               const cs_screen_cfg *cfg, cs_survivor *out, uint64_t cap, uint64_t *n_out,
               uint64_t *screens);
 
+/* No counterpart in the reference.  cs_screen for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same survivors, but every device
+ * buffer comes from a static 64 MiB scratch array in the library's code
+ * object (one per GPU, held until the process ends), so the call makes no
+ * hipMalloc / hipFree.  Record buffers hold at most 2^16 records (larger
+ * launches are split and redone).  Calls beyond the scratch size fail with
+ * CS_E_CAP before any scoring launch: the speculative-walk path needs about
+ * 7.5 bytes per 1000 candidates plus 20 MB (2.65e9 candidates: 40 MB, the
+ * limit is about 6e9); the transfer-map path (rates below 0.004, or forced)
+ * also 512 bytes per 65536-draw block, so about 9e7 candidates at most.
+ * Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU race.  Host
+ * bookkeeping (heaps, record replay) still uses the C++ heap. */
+int cs_screen_noalloc(cs_ctx *ctx, const cs_model *model, const cs_genome *elites, int ne,
+                      const cs_screen_cfg *cfg, cs_survivor *out, uint64_t cap, uint64_t *n_out, uint64_t *screens);
+
 int cs_get_stats(cs_ctx *ctx, cs_screen_stats *out);
 
 /* Testing hook, no counterpart in the reference: records per record buffer
