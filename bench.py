@@ -41,6 +41,10 @@ PEAK_FP32_TFLOPS = 157.3
 PUBLISHED_CPU = 17.2e6    # reference Zig SIMD screen, 4 threads (docs/SCALE.md:26)
 # measured once with rocprofv3 (profiles/screen_walk_offsets_2p65e9.md):
 # k_score VALU lane-instructions per candidate and HBM bytes per launch
+# (the profile names the source it was measured on; the line reports whether
+# that source is still the one built)
+PROFILED_SOURCE = "codec-arena_amd/csrc/screen_device.hip"
+PROFILED_SHA16 = "unprofiled"
 VALU_LANE_INSTR = 2139
 VALU_PEAK = 256 * 128 * 2.4e9  # lane-instructions/s: 256 CUs x 128 lanes/cycle x 2.4 GHz
 SCORE_TRAFFIC_PER_LAUNCH = 97.7e6 / 11
@@ -211,18 +215,26 @@ def mlp_line(scorer, steps, dt, stats, train_info, rates=None):
                           "holdout": train_info["holdout"], "rerank": "fp64 top-256 on the host"}}
 
 
-def cpu_baseline(sample, workers):
-    """The oracle (C restatement of src/screen.zig) on the host cores."""
+def cpu_baseline(sample, threads):
+    """The oracle's restatement of src/screen.zig in the reference's own form
+    (one host thread per RNG stream, the 16-wide vector score of
+    src/screen.zig:97-112, inlined splitmix64) on `threads` host threads."""
     from oracle_binding import Oracle
     o = Oracle()
     golden = os.path.join(REPO, "tests", "golden")
     m = o.load_model(os.path.join(golden, "c0_model.txt"))
     el = o.load_elites(os.path.join(golden, "c1_elites.txt"))
     t0 = time.perf_counter()
-    o.screen(m, el, sample, KEEP, 0, THREADS, sigma_of(0), float(np.float32(JUMP)), float(np.float32(IMM)),
-             workers=workers)
+    o.screen(m, el, sample, KEEP, 0, threads, sigma_of(0), float(np.float32(JUMP)), float(np.float32(IMM)),
+             workers=threads, simd=True)
     dt = time.perf_counter() - t0
     return sample / dt, dt
+
+
+def source_sha(rel):
+    import hashlib
+    with open(os.path.join(REPO, rel), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
 
 
 def cpu_model():
@@ -240,8 +252,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 26)
-    ap.add_argument("--cpu-workers", type=int, default=4)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 27,
+                    help="candidates of the 4-thread CPU leg (the all-threads leg scales it by threads / 4)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mlp", action="store_true", help="headline line without the learned-surrogate funnel")
     ap.add_argument("--funnel-gens", type=int, default=2,
@@ -339,7 +351,12 @@ def main():
                                     "achieved": LAMBDA * args.steps / (ms_score * 1e-3) * VALU_LANE_INSTR,
                                     "peak": VALU_PEAK, "unit": "lane-instr/s",
                                     "frac": LAMBDA * args.steps / (ms_score * 1e-3) * VALU_LANE_INSTR / VALU_PEAK,
-                                    "probe": valu_model(rates, LAMBDA * args.steps / (ms_score * 1e-3))},
+                                    "probe": valu_model(rates, LAMBDA * args.steps / (ms_score * 1e-3)),
+                                    "memtime_ghz": rates["memtime_ghz"]},
+                     "profiled_source": {"file": PROFILED_SOURCE, "sha16": PROFILED_SHA16,
+                                         "current_sha16": source_sha(PROFILED_SOURCE),
+                                         "stale": source_sha(PROFILED_SOURCE) != PROFILED_SHA16,
+                                         "fields": ["traffic", "valu_issue.lane_instr_per_candidate"]},
                      "flop_per_candidate": FLOP_PER_CANDIDATE, "launches": launches,
                      "avg_launch_ms": ms_score / max(1, launches), "k_map_ms_per_step": ms_map / args.steps,
                      "walk_fixes": sum(s.get("walk_fixes", 0) for s in stats),
@@ -355,10 +372,22 @@ def main():
     if funnel is not None:
         out["funnel_oracle_verified"] = funnel
     if not args.no_cpu:
-        v, secs = cpu_baseline(args.cpu_sample, args.cpu_workers)
-        out["cpu_baseline"] = {"value": v, "unit": "candidates/s", "cores": args.cpu_workers, "kind": "port",
-                               "sample": f"{args.cpu_sample} candidates of generation 0 (same elites, model, "
-                                         f"4 streams) in {secs:.1f} s on {cpu_model()}"}
+        legs = {}
+        nthr = max(1, min(os.cpu_count() or 1, 16))  # the GPU host's share: at most 16 threads
+        for t in sorted({THREADS, nthr}):
+            sample = args.cpu_sample * max(1, t // THREADS)
+            v, secs = cpu_baseline(sample, t)
+            legs[t] = {"value": v, "unit": "candidates/s", "cores": t,
+                       "sample": f"{sample} candidates of generation 0 (same elites and model, {t} streams) in "
+                                 f"{secs:.1f} s"}
+            print(f"cpu baseline, {t} threads: {v:.3e} candidates/s", file=sys.stderr, flush=True)
+        best = legs[nthr]
+        out["cpu_baseline"] = {"value": best["value"], "unit": "candidates/s", "cores": nthr, "kind": "port",
+                               "sample": best["sample"] + f" on {cpu_model()}",
+                               "form": "oracle/screen_oracle.c or_screen_simd: one thread per RNG stream, "
+                                       "16-wide vector score as src/screen.zig:97-112",
+                               "threads_4": legs[THREADS],
+                               "published_reference_threads_4": PUBLISHED_CPU}
     print(json.dumps(out), flush=True)
 
 
