@@ -81,6 +81,24 @@ struct DevModel {
 };
 typedef const __attribute__((address_space(4))) DevModel *ConstModel;
 
+// The quadratic model as the LDS-elites scorer keeps it in LDS, in the order
+// the score reads it: w0, then for each lane j = 0..15 the run lin[j],
+// w(j,j), w(j+1,j), ..., w(15,j) (17 - j values), then lin[16..28].
+constexpr int LDS_MODEL_FLOATS = 1 + 152 + 13;
+constexpr int LDS_MODEL_BYTES = (LDS_MODEL_FLOATS * 4 + 15) / 16 * 16;
+__host__ __device__ constexpr int lds_model_run(int j) { return 1 + 17 * j - j * (j - 1) / 2; }
+// DevModel index of LDS model slot i
+__device__ inline int lds_model_src(int i) {
+    if (i == 0) return 0;  // w0
+    if (i >= 1 + 152) return 1 + 16 + (i - 153);  // lin[16..28]
+    int j = 0;
+    while (j < 15 && lds_model_run(j + 1) <= i) ++j;
+    const int r = i - lds_model_run(j);  // 0: lin[j]; r >= 1: w(j + r - 1, j)
+    if (r == 0) return 1 + j;
+    const int row = j + r - 1;
+    return 1 + CS_NTOT + row * (row + 1) / 2 + j;
+}
+
 // ------------------------------------------------------------ splitmix64
 
 // state after n increments is seed + n*gamma; draw n is mix(that).  The
@@ -109,10 +127,22 @@ __device__ __forceinline__ uint32_t draw_hi(uint64_t s) {
 }
 // (out >> 40): the 24-bit unit numerator
 __device__ __forceinline__ uint32_t draw_u24(uint64_t s) { return mix_zhi(mix_mid(s)) >> 8; }
-// (out & 1): crossover bit; needs only the low word of z
-__device__ __forceinline__ uint32_t draw_bit(uint64_t s) {
-    const uint32_t l = mix_mid(s).lo * (uint32_t)MIX2;
-    return (l ^ (l >> 31)) & 1u;
+// (out & 1): crossover bit, = bit0(z) ^ bit31(z) with z = u * MIX2.  MIX2 is
+// odd, so bit0(z) = bit0(u.lo), and adding u.lo 2^31 (mod 2^32) to the low
+// word of z flips its bit 31 exactly when bit0(u.lo) = 1: the bit is bit 31
+// of u.lo * (MIX2 + 2^31) mod 2^32, i.e. the sign of one 32-bit product.
+__device__ __forceinline__ int32_t draw_bit_signed(uint64_t s) {
+    return (int32_t)(mix_mid(s).lo * (uint32_t)(MIX2 + 0x80000000ull));
+}
+__device__ __forceinline__ uint32_t draw_bit(uint64_t s) { return (uint32_t)draw_bit_signed(s) >> 31; }
+// (out >> 40) - 2^23 as a signed integer: the unit numerator m centred.
+// (zhi + 2^31) >> 8, arithmetic, is m - 2^23; the 2^31 rides in the 64-bit
+// addend of the product that forms the high word.
+__device__ __forceinline__ int32_t draw_centered(uint64_t s) {
+    const Mid u = mix_mid(s);
+    const uint64_t p = (uint64_t)u.lo * (uint32_t)MIX2 + 0x8000000000000000ull;
+    const uint32_t zb = (uint32_t)(p >> 32) + u.lo * (uint32_t)(MIX2 >> 32) + u.hi * (uint32_t)MIX2;
+    return (int32_t)zb >> 8;
 }
 __device__ __forceinline__ float unit_f(uint32_t m) { return (float)m * 0x1p-24f; }
 
@@ -631,27 +661,57 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
 // Genes [K0, K1) of one candidate.  s0 is the stream state of the first gene
 // draw; gene k uses draws s0 + 2k (crossover bit) and s0 + 2k + 1 (mutation
 // unit), or s0 + k (immigrant unit).
-template <bool IMM, bool FAST_MUT, int K0, int K1>
+typedef const __attribute__((address_space(3))) float *LdsRow;
+
+// Gene k of parent a or b, picked by the sign of the crossover draw.  With
+// the elites in LDS the two rows' addresses are made opaque, so the pick is
+// one select of a 32-bit LDS address and the gene offset folds into the
+// ds_read (the compiler otherwise selects the elite index and multiplies it
+// by the row length for every gene).
+template <bool LDS>
+struct ParentRows {
+    LdsRow la, lb;
+    const float *ga, *gb;
+    __device__ __forceinline__ ParentRows(const float *el, uint32_t ia, uint32_t ib) {
+        if constexpr (LDS) {
+            uint32_t oa = (uint32_t)(uintptr_t)(LdsRow)el + ia * (uint32_t)(CS_NTOT * 4);
+            uint32_t ob = (uint32_t)(uintptr_t)(LdsRow)el + ib * (uint32_t)(CS_NTOT * 4);
+            asm volatile("" : "+v"(oa), "+v"(ob));
+            la = (LdsRow)(uintptr_t)oa;
+            lb = (LdsRow)(uintptr_t)ob;
+        } else {
+            ga = el + ia * CS_NTOT;
+            gb = el + ib * CS_NTOT;
+        }
+    }
+    __device__ __forceinline__ float pick(int32_t bs, int k) const {
+        if constexpr (LDS) return (bs < 0 ? lb : la)[k];
+        else return (bs < 0 ? gb : ga)[k];
+    }
+};
+
+template <bool IMM, bool FAST_MUT, int K0, int K1, bool LDS = false>
 __device__ __forceinline__ void synth_genes(float *g, uint64_t s0, const float *el, uint32_t ia, uint32_t ib,
                                             float sigma, float sig24) {
     if (IMM) {
 #pragma unroll
         for (int k = K0; k < K1; ++k) g[k] = unit_f(draw_u24(s0 + (uint64_t)k * GAMMA));
     } else {
+        const ParentRows<LDS> rows(el, ia, ib);
 #pragma unroll
         for (int k = K0; k < K1; ++k) {
-            const uint32_t bit = draw_bit(s0 + (uint64_t)(2 * k) * GAMMA);
-            const uint32_t mu = draw_u24(s0 + (uint64_t)(2 * k + 1) * GAMMA);
-            const uint32_t e = bit ? ib : ia;
-            const float par = el[e * CS_NTOT + k];
+            const int32_t bs = draw_bit_signed(s0 + (uint64_t)(2 * k) * GAMMA);
+            const float par = rows.pick(bs, k);
             if (FAST_MUT) {
-                // ((u - 0.5) * 2) * sigma with u = mu 2^-24 equals
-                // (2 mu - 2^24) * (sigma 2^-24) before the one rounding when
-                // sigma 2^-24 is exact; the clamp of a finite sum to [0, 1]
+                // ((u - 0.5) * 2) * sigma with u = m 2^-24 has the real value
+                // (m - 2^23) * (sigma 2^-23), which is rounded once when
+                // sigma 2^-23 is exact; the clamp of a finite sum to [0, 1]
                 // is one median-of-three.
-                const float d = (float)((int32_t)(mu << 1) - (1 << 24)) * sig24;
+                const int32_t c = draw_centered(s0 + (uint64_t)(2 * k + 1) * GAMMA);
+                const float d = (float)c * (sig24 * 2.0f);
                 g[k] = __builtin_amdgcn_fmed3f(par + d, 0.0f, 1.0f);
             } else {
+                const uint32_t mu = draw_u24(s0 + (uint64_t)(2 * k + 1) * GAMMA);
                 float d = (unit_f(mu) - 0.5f) * 2.0f;
                 d = d * sigma;
                 const float v = par + d;
@@ -699,7 +759,7 @@ __device__ __attribute__((noinline)) void emit_mlp_records(Record *rec, uint32_t
 // IMM: 0 normal candidates, 1 immigrants, 2 either, by the uniform imm_rt
 // (one inlined copy of the scorer for both kinds: the MLP variants).
 template <int IMM, bool LDS_ELITES, bool FAST_MUT, int SC, bool DUMP>
-__device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml,
+__device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml, const float *mdl,
                                             const float *el, const uint32_t *q, uint32_t head,
                                             uint32_t count, uint64_t bstart, uint64_t first,
                                             uint64_t seed, uint32_t stream, float bound,
@@ -721,18 +781,22 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
     // no gene stays live across the network.
     constexpr int KH = SC == SC_QUAD ? CS_NTOT : MLP_IN;
     float g[CS_NTOT];
-    if (imm) synth_genes<true, FAST_MUT, 0, KH>(g, s0, el, ia, ib, a.sigma, a.sig24);
-    else synth_genes<false, FAST_MUT, 0, KH>(g, s0, el, ia, ib, a.sigma, a.sig24);
+    if (imm) synth_genes<true, FAST_MUT, 0, KH, LDS_ELITES>(g, s0, el, ia, ib, a.sigma, a.sig24);
+    else synth_genes<false, FAST_MUT, 0, KH, LDS_ELITES>(g, s0, el, ia, ib, a.sigma, a.sig24);
     // jump test draw at s0 + ngene draws (already known), then index and value
     const uint64_t sj = s0 + (uint64_t)(imm ? CS_NTOT : 2 * CS_NTOT) * GAMMA;
     uint32_t gi = CS_NTOT;
     float jval = 0.0f;
-    if (jp) {
-        gi = draw_hi(sj + GAMMA) % (uint32_t)CS_NTOT;
-        jval = unit_f(draw_u24(sj + 2 * GAMMA));
-    }
+    // jumps are rare (jump_p per candidate): the gene replacement runs only
+    // in batches where some lane jumped (a wave-uniform branch)
+    if (__ballot(jp)) {
+        if (jp) {
+            gi = draw_hi(sj + GAMMA) % (uint32_t)CS_NTOT;
+            jval = unit_f(draw_u24(sj + 2 * GAMMA));
+        }
 #pragma unroll
-    for (int k = 0; k < KH; ++k) g[k] = (k == (int)gi) ? jval : g[k];
+        for (int k = 0; k < KH; ++k) g[k] = (k == (int)gi) ? jval : g[k];
+    }
     float score;
     if constexpr (SC != SC_QUAD) {
         bf16x8 x0, x1;
@@ -753,10 +817,30 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
     // fetched per batch through an opaque uniform pointer so they stream
     // through scalar loads instead of being pinned (and spilled) for the
     // whole kernel.
+    float S = 0.0f, r = 0.0f;
+    if constexpr (LDS_ELITES) {
+        // broadcast LDS reads of this lane run; the address is re-opaqued
+        // per run so that only one run's weights are live at a time
+        float S2 = 0.0f;
+        uint32_t ma = (uint32_t)(uintptr_t)mdl;
+        asm volatile("" : "+v"(ma));
+        const LdsRow w = (LdsRow)(uintptr_t)ma;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const float acc = w[0] + w[lds_model_run(j)] * g[j];
+            float sum = 0.0f;
+#pragma unroll
+            for (int i = j; i < 16; ++i) sum = sum + w[lds_model_run(j) + 1 + (i - j)] * (g[j] * g[i]);
+            const float ln = acc + sum;
+            S2 = (j == 0) ? ln : S2 + ln;
+        }
+#pragma unroll
+        for (int i = 16; i < CS_NTOT; ++i) r = r + w[153 + (i - 16)] * g[i];
+        S = S2;
+    } else {
     uint64_t mp = (uint64_t)a.model;
     asm volatile("" : "+s"(mp));
     ConstModel md = (ConstModel)mp;
-    float S = 0.0f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const float acc = md->w0 + md->lin[j] * g[j];
@@ -766,9 +850,9 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
         const float ln = acc + sum;
         S = (j == 0) ? ln : S + ln;
     }
-    float r = 0.0f;
 #pragma unroll
     for (int i = 16; i < CS_NTOT; ++i) r = r + md->lin[i] * g[i];
+    }
     score = S + r;
     }
     const bool pass = act && !(score <= bound);
@@ -820,11 +904,20 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
     constexpr int NW = score_waves<SC>();
     extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
     uint32_t *qbase = smem;  // NW waves x 2 rings x QCAP
-    float *el_lds = reinterpret_cast<float *>(smem + NW * 2 * QCAP);
+    // quadratic scorer with elites in LDS: the model (lane-run order) first
+    constexpr bool LDS_MODEL = LDS_ELITES && SC == SC_QUAD;
+    float *mdl_lds = reinterpret_cast<float *>(smem + NW * 2 * QCAP);
+    float *el_lds = mdl_lds + (LDS_MODEL ? LDS_MODEL_BYTES / 4 : 0);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const float *el = a.elites;
+    const float *mdl = mdl_lds;
     if (LDS_ELITES) {
         for (uint32_t i = threadIdx.x; i < a.ne * CS_NTOT; i += blockDim.x) el_lds[i] = a.elites[i];
+        if (LDS_MODEL) {
+            const float *src = reinterpret_cast<const float *>(a.model);
+            for (uint32_t i = threadIdx.x; i < (uint32_t)LDS_MODEL_FLOATS; i += blockDim.x)
+                mdl_lds[i] = src[lds_model_src((int)i)];
+        }
         __syncthreads();
         el = el_lds;
     }
@@ -938,13 +1031,13 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             while (c < bend && idx < st.my_lambda) {
                 enqueue();
                 while (tailN - headN >= 64) {
-                    score_batch<0, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qN, headN, 64, bstart, first, st.seed,
+                    score_batch<0, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, qN, headN, 64, bstart, first, st.seed,
                                                                    (uint32_t)t, sg.bound, lane);
                     headN += 64;
                     screened += 64;
                 }
                 if (tailI - headI >= 64) {
-                    score_batch<1, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qI, headI, 64, bstart, first, st.seed,
+                    score_batch<1, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, qI, headI, 64, bstart, first, st.seed,
                                                                    (uint32_t)t, sg.bound, lane);
                     headI += 64;
                     screened += 64;
@@ -952,12 +1045,12 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
                 __builtin_amdgcn_wave_barrier();
             }
             if (tailN > headN) {
-                score_batch<0, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qN, headN, tailN - headN, bstart, first,
+                score_batch<0, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, qN, headN, tailN - headN, bstart, first,
                                                                st.seed, (uint32_t)t, sg.bound, lane);
                 screened += tailN - headN;
             }
             if (tailI > headI) {
-                score_batch<1, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, qI, headI, tailI - headI, bstart, first,
+                score_batch<1, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, qI, headI, tailI - headI, bstart, first,
                                                                st.seed, (uint32_t)t, sg.bound, lane);
                 screened += tailI - headI;
             }
@@ -979,7 +1072,7 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
                 else if (tailN > headN) { im = false; cnt = tailN - headN; }
                 else if (tailI > headI) { im = true; cnt = tailI - headI; }
                 else break;
-                score_batch<2, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, el, im ? qI : qN, im ? headI : headN, cnt,
+                score_batch<2, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, im ? qI : qN, im ? headI : headN, cnt,
                                                                bstart, first, st.seed, (uint32_t)t, sg.bound, lane,
                                                                im);
                 if (im) headI += cnt;
@@ -1570,7 +1663,7 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     const size_t el_bytes = (size_t)ne * CS_NTOT * 4;
     h.lds_elites = sc == SC_QUAD && el_bytes <= 48 * 1024 && h.sa.dump_genes == nullptr;
     if (sc == SC_QUAD) {
-        h.smem = score_waves<SC_QUAD>() * 2 * QCAP * 4 + (h.lds_elites ? el_bytes : 0);
+        h.smem = score_waves<SC_QUAD>() * 2 * QCAP * 4 + (h.lds_elites ? LDS_MODEL_BYTES + el_bytes : 0);
     } else {
         const size_t nw = score_waves<SC_MLP_BF16>();
         h.smem = nw * 2 * QCAP * 4 + (sc == SC_MLP_FP8 ? MLP_W2_FP8_BYTES : MLP_W2_BF16_BYTES) + MLP_VEC_BYTES +
