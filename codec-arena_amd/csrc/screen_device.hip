@@ -30,6 +30,7 @@
 #include <type_traits>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -320,7 +321,10 @@ __global__ __launch_bounds__(64) void k_down(const uint64_t *__restrict__ in,
 
 // --------------------------------------------------------------- k_walk
 
-// Segments of 2^25 draws (512 blocks).  The true candidate chain of a stream
+// Segments of 2^24 draws (256 blocks) by default: the walk is close to the
+// VALU issue limit, and on the production shard 2^24 beat 2^25 (9.2 vs
+// 10.7 ms: twice the waves, 1.125x the work) and 2^23 (10.4 ms: 1.2x more
+// burn-in work).  The true candidate chain of a stream
 // is found segment by segment in parallel: each wavefront starts a chain at a
 // guessed start BURN draws before its segment and follows it with ballot
 // speculation; two chains that ever share a start coincide from then on, and
@@ -329,9 +333,10 @@ __global__ __launch_bounds__(64) void k_down(const uint64_t *__restrict__ in,
 // the true chain iff its first start >= segment start (x_s) equals the first
 // start >= that point found by segment s-1 (y_{s-1}); segment 0 starts at the
 // true stream start.  Unverified segments are re-walked from y_{s-1}.
-constexpr int SEG_LOG = 25;
-constexpr uint64_t SEG = 1ull << SEG_LOG;
-constexpr int SEG_BLOCKS_LOG = SEG_LOG - BLOCK_LOG;  // 512 blocks per segment
+// Segment length: 2^seg_log draws (2^(seg_log - 16) blocks), a context
+// setting (cs_ctx::walk_seg_log; CSFUNNEL_WALK_SEG_LOG in [18, 30] at
+// context creation overrides the default, for measurement).
+constexpr int SEG_LOG_DEFAULT = 24;
 constexpr uint64_t BURN = 1ull << 22;  // ~14x the mean merge distance
 
 struct SegOut {
@@ -400,19 +405,21 @@ __device__ void walk_segment(uint64_t seed, uint64_t c, uint64_t seg_start, uint
 __global__ __launch_bounds__(64) void k_walk(const StreamInfo *__restrict__ streams,
                                              const uint64_t *__restrict__ seg_first, int T, uint64_t nseg,
                                              uint32_t immT, uint32_t jumpT, uint64_t *__restrict__ entries,
-                                             SegOut *__restrict__ segout, int64_t fix_flat, uint64_t fix_start) {
+                                             SegOut *__restrict__ segout, int64_t fix_flat, uint64_t fix_start,
+                                             int seg_log) {
     const int lane = threadIdx.x;
     for (uint64_t f = blockIdx.x; f < nseg; f += gridDim.x) {
         const uint64_t ff = fix_flat >= 0 ? (uint64_t)fix_flat : f;
         const int t = find_seg(seg_first, T, ff);
         const uint64_t s = ff - seg_first[t];
         const StreamInfo st = streams[t];
-        const uint64_t seg_start = s << SEG_LOG;
-        const uint64_t seg_end = min(seg_start + SEG, st.nblocks << BLOCK_LOG);
+        const uint64_t seg_start = s << seg_log;
+        const uint64_t seg_end = min(seg_start + (1ull << seg_log), st.nblocks << BLOCK_LOG);
         uint64_t c;
         if (fix_flat >= 0) c = fix_start;
-        else c = s == 0 ? 0 : seg_start - BURN;
-        walk_segment(st.seed, c, seg_start, seg_end, immT, jumpT, entries + st.blk_first + (s << SEG_BLOCKS_LOG),
+        else c = seg_start > BURN ? seg_start - BURN : 0;  // 0: the true stream start
+        walk_segment(st.seed, c, seg_start, seg_end, immT, jumpT,
+                     entries + st.blk_first + (s << (seg_log - BLOCK_LOG)),
                      segout + ff, lane);
         if (fix_flat >= 0) break;
     }
@@ -421,11 +428,11 @@ __global__ __launch_bounds__(64) void k_walk(const StreamInfo *__restrict__ stre
 // entries[b] += segment base index (per stream prefix of segment totals)
 __global__ void k_rebase(const StreamInfo *__restrict__ streams, const uint64_t *__restrict__ blk_first, int T,
                          uint64_t nblk, const uint64_t *__restrict__ seg_first,
-                         const uint64_t *__restrict__ seg_base, uint64_t *__restrict__ entries) {
+                         const uint64_t *__restrict__ seg_base, uint64_t *__restrict__ entries, int seg_blocks_log) {
     for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nblk; f += (uint64_t)gridDim.x * blockDim.x) {
         const int t = find_seg(blk_first, T, f);
         const uint64_t b = f - blk_first[t];
-        entries[f] += seg_base[seg_first[t] + (b >> SEG_BLOCKS_LOG)] << 8;
+        entries[f] += seg_base[seg_first[t] + (b >> seg_blocks_log)] << 8;
     }
 }
 
@@ -1438,10 +1445,11 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
     const int T = h.T;
     int rc;
     *ok = false;
+    const int seg_log = ctx->walk_seg_log, seg_blocks_log = seg_log - BLOCK_LOG;
     std::vector<uint64_t> seg_first(T + 1, 0);
     for (int t = 0; t < T; ++t) {
         const uint64_t nb = h.si[t].nblocks;
-        seg_first[t + 1] = seg_first[t] + ((nb + (1ull << SEG_BLOCKS_LOG) - 1) >> SEG_BLOCKS_LOG);
+        seg_first[t + 1] = seg_first[t] + ((nb + (1ull << seg_blocks_log) - 1) >> seg_blocks_log);
     }
     const uint64_t nseg = seg_first[T];
     if ((rc = acquire(h, h.B->seg_first_buf, sizeof(uint64_t) * (T + 1)))) return rc;
@@ -1453,7 +1461,7 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
         unsigned grid = (unsigned)std::min<uint64_t>(nseg, 1u << 20);
         hipLaunchKernelGGL(k_walk, dim3(grid), dim3(64), 0, ctx->stream, (const StreamInfo *)h.B->streams.p,
                            (const uint64_t *)h.B->seg_first_buf.p, T, nseg, immT, jumpT, (uint64_t *)h.B->entries.p,
-                           (SegOut *)h.B->segout.p, (int64_t)-1, (uint64_t)0);
+                           (SegOut *)h.B->segout.p, (int64_t)-1, (uint64_t)0, seg_log);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -1471,7 +1479,8 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
             if (++fixes > 32) return CS_OK;  // *ok stays false
             hipLaunchKernelGGL(k_walk, dim3(1), dim3(64), 0, ctx->stream, (const StreamInfo *)h.B->streams.p,
                                (const uint64_t *)h.B->seg_first_buf.p, T, (uint64_t)1, immT, jumpT,
-                               (uint64_t *)h.B->entries.p, (SegOut *)h.B->segout.p, (int64_t)f, so[f - 1].y);
+                               (uint64_t *)h.B->entries.p, (SegOut *)h.B->segout.p, (int64_t)f, so[f - 1].y,
+                               seg_log);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(&so[f], (SegOut *)h.B->segout.p + f, sizeof(SegOut), hipMemcpyDeviceToHost,
                                   ctx->stream));
@@ -1494,7 +1503,7 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
         unsigned grid = (unsigned)std::min<uint64_t>((nblk + 255) / 256, 1u << 16);
         hipLaunchKernelGGL(k_rebase, dim3(grid), dim3(256), 0, ctx->stream, (const StreamInfo *)h.B->streams.p,
                            (const uint64_t *)h.B->blk_first.p, T, nblk, (const uint64_t *)h.B->seg_first_buf.p,
-                           (const uint64_t *)h.B->seg_base.p, (uint64_t *)h.B->entries.p);
+                           (const uint64_t *)h.B->seg_base.p, (uint64_t *)h.B->entries.p, seg_blocks_log);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
@@ -1751,6 +1760,11 @@ int cs_ctx_create(int device, cs_ctx **out) {
     cs_ctx *c = new cs_ctx();
     c->device = device;
     c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    c->walk_seg_log = SEG_LOG_DEFAULT;
+    if (const char *e = std::getenv("CSFUNNEL_WALK_SEG_LOG")) {
+        const int v = std::atoi(e);
+        if (v >= 18 && v <= 30) c->walk_seg_log = v;
+    }
     if (int rc = scratch_address(device, &c->scratch)) {
         delete c;
         return rc;
