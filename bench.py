@@ -155,6 +155,31 @@ def mlp_recall(dev, elites, scorer, mlp):
     return raw, len(top & set(_rows(fun, RECALL_LAMBDA))) / 64
 
 
+def production_recall(dev, elites, mlp, scorers):
+    """Top-64 recall at the production size: the exact FP64 top 64 of the MLP
+    over all 2.65e9 candidates of generation 0 (the device FP64 scorer, no
+    dump) against each reduced-precision screen's top 64, raw and after the
+    funnel's FP64 re-rank of its top 256, identified by (stream, index)."""
+    from csfunnel.mlp import screen_ex, screen_f64, screen_funnel
+    cfg = _cfg(LAMBDA, 64, 0)
+    t0 = time.perf_counter()
+    ref, s64, n = screen_f64(dev, elites, cfg, mlp)
+    f64_s = time.perf_counter() - t0
+    assert n == LAMBDA and len(ref) == 64
+    key = lambda a: set(zip(a["stream"].tolist(), a["index"].tolist()))  # noqa: E731
+    top = key(ref)
+    out = {"lambda": LAMBDA, "k": 64, "fp64_screen_s": round(f64_s, 3),
+           "fp64_cut_gap": float(s64[62] - s64[63]), "reference": "exact FP64 top 64 (device FP64 scorer)"}
+    for name, sc in scorers:
+        raw, _, _, _ = screen_ex(dev, None, elites, cfg, sc, mlp)
+        fun, _ = screen_funnel(dev, elites, cfg, sc, mlp, rerank=4)
+        out[name] = len(top & key(raw)) / 64
+        out[name + "_after_rerank"] = len(top & key(fun)) / 64
+        print(f"recall at 2.65e9, {name}: {out[name]:.4f} raw, {out[name + '_after_rerank']:.4f} after re-rank "
+              f"(FP64 reference screen {f64_s:.1f} s)", file=sys.stderr, flush=True)
+    return out
+
+
 def _cfg(lam, keep, gen):
     from csfunnel import screen as scr
     return scr.Cfg(lam, keep, gen, THREADS, sigma_of(gen), float(np.float32(JUMP)), float(np.float32(IMM)))
@@ -279,6 +304,7 @@ def main():
         else:
             train_info.pop("genes")
         dt, stats = run_generations(dev, model, elites, scorer, mlp, args.steps, args.warmup, 64)
+        prod = production_recall(dev, elites, mlp, [(args.scorer, scorer)])
         raw, rr = mlp_recall(dev, elites, scorer, mlp)
         rates = scr.valu_rates(0)
         dev.close()
@@ -289,7 +315,9 @@ def main():
                "scaling": "weak", "vs_baseline": None, "dtype": ml.pop("dtype"), "data": "synthetic",
                "config": {"workload": "full funnel: 2.65e9 candidates/generation scored by the 16-256-256-1 MLP, "
                                       "keep 64, 128 elites, 4 streams", "lambda": LAMBDA, "keep": 64},
-               "topk_recall_vs_fp64": {"screen": raw, "after_rerank": rr, "k": 64, "lambda": RECALL_LAMBDA},
+               "topk_recall_vs_fp64": {"production": prod,
+                                       "dumped_2^22": {"screen": raw, "after_rerank": rr, "k": 64,
+                                                       "lambda": RECALL_LAMBDA}},
                **ml}
         print(json.dumps(out), flush=True)
         return
@@ -297,13 +325,14 @@ def main():
     dt, stats = run_generations(dev, model, elites, 0, None, args.steps, args.warmup, KEEP)
     value = LAMBDA * args.steps / dt
     rates = scr.valu_rates(0)
-    recall = {"quad_fp32": quad_recall(dev, model, elites), "k_quad": KEEP, "lambda": RECALL_LAMBDA}
+    recall = {"quad_fp32": quad_recall(dev, model, elites), "k_quad": KEEP, "lambda_dumped": RECALL_LAMBDA}
     mlp_out = None
     funnel = None
     if not args.no_mlp:
         mlp, wq, train_info = train_surrogate(dev)
         genes = train_info.pop("genes")
         mdt, mstats = run_generations(dev, model, elites, 1, mlp, 2, 1, 64)
+        prod = production_recall(dev, elites, mlp, [("mlp_bf16", 1)])
         raw, rr = mlp_recall(dev, elites, 1, mlp)
         recall.update({"mlp_bf16": raw, "mlp_bf16_after_rerank": rr, "k_mlp": 64})
         mlp_out = mlp_line(1, 2, mdt, mstats, train_info, rates)
@@ -312,6 +341,9 @@ def main():
         fdt, fstats = run_generations(dev, model, elites, 2, mlp, 2, 1, 64)
         fraw, frr = mlp_recall(dev, elites, 2, mlp)
         recall.update({"mlp_fp8": fraw, "mlp_fp8_after_rerank": frr})
+        fprod = production_recall(dev, elites, mlp, [("mlp_fp8", 2)])
+        prod.update({k: v for k, v in fprod.items() if k.startswith("mlp_fp8")})
+        recall["mlp_production"] = prod
         fp8_line = mlp_line(2, 2, fdt, fstats, train_info, rates)
         mlp_out["fp8"] = {k: fp8_line[k] for k in ("value", "unit", "ms_per_step", "dtype", "roofline")}
         if args.funnel_gens > 0:

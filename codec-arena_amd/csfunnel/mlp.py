@@ -19,12 +19,12 @@ from .screen import SURVIVOR_DTYPE, Cfg, Model, Screen
 
 N_PARAMS = 70401
 IN, H = 16, 256
-SCORER_QUAD, SCORER_MLP_BF16, SCORER_MLP_FP8 = 0, 1, 2
+SCORER_QUAD, SCORER_MLP_BF16, SCORER_MLP_FP8, SCORER_MLP_F64 = 0, 1, 2, 3
 
 
 class ScreenOpts(C.Structure):
     _fields_ = [("scorer", C.c_int), ("_pad", C.c_int), ("mlp", C.c_void_p), ("dump_genes", C.c_void_p),
-                ("dump_scores", C.c_void_p)]
+                ("dump_scores", C.c_void_p), ("scores64", C.c_void_p)]
 
 
 def unpack(p: np.ndarray):
@@ -148,9 +148,11 @@ class MLP:
 
 
 def screen_ex(dev: Screen, model: Optional[Model], elites, cfg: Cfg, scorer: int, mlp: Optional[MLP] = None,
-              dump: bool = False):
+              dump: bool = False, keys: Optional[np.ndarray] = None):
     """cs_screen_ex: survivors, screens_count and (dump) every candidate's
-    genes [lambda, 29] and score [lambda] as cuda tensors in stream order."""
+    genes [lambda, 29] and score [lambda] as cuda tensors in stream order.
+    keys (float64 [keep]) receives each survivor's ranking key: the FP64
+    score for SCORER_MLP_F64."""
     ne = len(elites)
     earr = elites if isinstance(elites, C.Array) else (Genome * max(1, ne))(*elites)
     cap = max(1, int(cfg.keep))
@@ -161,13 +163,24 @@ def screen_ex(dev: Screen, model: Optional[Model], elites, cfg: Cfg, scorer: int
         dg = torch.empty((cfg.lambda_, 29), dtype=torch.float32, device="cuda")
         ds = torch.empty(cfg.lambda_, dtype=torch.float32, device="cuda")
         torch.cuda.synchronize()
+    if keys is not None:
+        assert keys.dtype == np.float64 and keys.flags.c_contiguous and keys.size >= cap
     opts = ScreenOpts(scorer, 0, mlp._h if mlp is not None else None, dg.data_ptr() if dump else None,
-                      ds.data_ptr() if dump else None)
+                      ds.data_ptr() if dump else None, keys.ctypes.data if keys is not None else None)
     c = cfg.to_c()
     check(lib().cs_screen_ex(dev._h, C.byref(model.raw) if model is not None else None, earr, ne, C.byref(c),
                              C.byref(opts), out.ctypes.data_as(C.POINTER(Survivor)), cap, C.byref(n_out),
                              C.byref(screens)))
     return out[: n_out.value].copy(), screens.value, dg, ds
+
+
+def screen_f64(dev: Screen, elites, cfg: Cfg, mlp: MLP):
+    """The exact top `keep` of the MLP evaluated in FP64 on the device over
+    every candidate of the generation (no dump): survivors, their FP64 scores
+    and screens_count."""
+    keys = np.zeros(max(1, int(cfg.keep)), dtype=np.float64)
+    surv, screens, _, _ = screen_ex(dev, None, elites, cfg, SCORER_MLP_F64, mlp, keys=keys)
+    return surv, keys[: len(surv)].copy(), screens
 
 
 def forward_f64(params: np.ndarray, genes: np.ndarray) -> np.ndarray:

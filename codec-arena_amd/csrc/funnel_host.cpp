@@ -174,26 +174,26 @@ static int slurp(const char *path, size_t limit, std::string &out) {
 // parent is strictly greater; full phase rejects score <= root, otherwise
 // replaces the root and sifts down toward the strictly smaller child (left
 // first).
-void StreamHeap::offer(const cs_survivor &s) {
+void StreamHeap::offer(const cs_survivor &s, double key) {
     if (slots.size() < k) {
-        slots.push_back(s);
+        slots.push_back(HeapItem{key, s});
         size_t i = slots.size() - 1;
         while (i > 0) {
             size_t up = (i - 1) >> 1;
-            if (slots[up].score <= slots[i].score) break;
+            if (slots[up].key <= slots[i].key) break;
             std::swap(slots[up], slots[i]);
             i = up;
         }
         return;
     }
-    if (s.score <= slots[0].score) return;
-    slots[0] = s;
+    if (key <= slots[0].key) return;
+    slots[0] = HeapItem{key, s};
     size_t i = 0;
     const size_t n = slots.size();
     for (;;) {
         size_t l = 2 * i + 1, r = l + 1, m = i;
-        if (l < n && slots[l].score < slots[m].score) m = l;
-        if (r < n && slots[r].score < slots[m].score) m = r;
+        if (l < n && slots[l].key < slots[m].key) m = l;
+        if (r < n && slots[r].key < slots[m].key) m = r;
         if (m == i) break;
         std::swap(slots[m], slots[i]);
         i = m;

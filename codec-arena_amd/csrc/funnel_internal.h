@@ -24,21 +24,28 @@ struct Record {
     uint32_t ib;     // elite index of parent b
     float g[CS_NTOT];
     uint32_t _pad;
+    double score64;  // the FP64 scorer's score (the others: score)
 };
-static_assert(sizeof(Record) == 152, "Record layout");
+static_assert(sizeof(Record) == 160, "Record layout");
 
-// Bounded min-heap of the reference (src/screen.zig:151-193), one per stream.
+// Bounded min-heap of the reference (src/screen.zig:151-193), one per
+// stream, ordered by a double key: the f32 score (exactly: the conversion
+// keeps order and equality) or the FP64 scorer's score.
+struct HeapItem {
+    double key;
+    cs_survivor s;
+};
 struct StreamHeap {
     uint64_t k = 0;
-    std::vector<cs_survivor> slots;  // slots.size() == n
+    std::vector<HeapItem> slots;  // slots.size() == n
     void reset(uint64_t keep) {
         k = keep;
         slots.clear();
         slots.reserve(keep < (1u << 20) ? keep : (1u << 20));
     }
     bool full() const { return slots.size() >= k; }
-    float root() const { return slots[0].score; }
-    void offer(const cs_survivor &s);
+    double root() const { return slots[0].key; }
+    void offer(const cs_survivor &s, double key);
 };
 
 // Zig {d:.4} of an f32; appends to `out`.

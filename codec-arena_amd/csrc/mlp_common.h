@@ -107,6 +107,10 @@ struct MlpScreenW {
     float w_scale;   // fp8: power of two; W2 was multiplied by it before conversion
     float descale;   // q / (h_scale * w_scale), q = GELU_Q: FP8 layer-2 accumulators to v = q z2
     float q1;        // v = x q1 in the layer-1 GELU: GELU_Q / h_scale (fp8) or GELU_Q
+    // FP64 scorer: W1 n-major [256][16], W2 as v_mfma_f64_16x16x4f64 A
+    // fragments [n-step 64][m-tile 16][lane 64], b1 | b2 | w3, b3
+    const double *w1d, *w2d, *vd;
+    double b3d;
 };
 
 }  // namespace csf

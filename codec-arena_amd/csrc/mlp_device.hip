@@ -406,7 +406,7 @@ int cs_mlp_create(cs_ctx *ctx, uint64_t seed, cs_mlp **out) {
 int cs_mlp_destroy(cs_mlp *m) {
     if (!m) return CS_OK;
     (void)hipStreamSynchronize(m->ctx->stream);
-    csf::DevBuf *bufs[] = {&m->params, &m->grads, &m->m, &m->v, &m->w1b, &m->w2b, &m->w1f, &m->w1fh, &m->w2f, &m->w2f8,
+    csf::DevBuf *bufs[] = {&m->params, &m->grads, &m->m, &m->v, &m->w1b, &m->w2b, &m->w1f, &m->w1fh, &m->w2f, &m->w2f8, &m->f64pack,
                            &m->xb, &m->z1, &m->h1b, &m->z2, &m->h2b, &m->out, &m->dout, &m->dz2f, &m->dz2b,
                            &m->dh1, &m->dz1f, &m->dz1b, &m->loss_part, &m->gws};
     for (auto *b : bufs)

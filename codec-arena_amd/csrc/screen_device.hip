@@ -73,6 +73,7 @@ struct LaunchSeg {  // one stream's block range in a scoring launch
     uint64_t b0;
     float bound;  // NaN: accept every candidate (heap not full yet)
     uint32_t _pad;
+    double bound64;  // the same bound for the FP64 scorer
 };
 
 struct DevModel {
@@ -472,9 +473,15 @@ constexpr size_t MLP_VEC_BYTES = 2 * MLP_H * 4 + (MLP_H / 2) * 4;  // b1 | b2 | 
 constexpr size_t MLP_WAVE_BYTES = 64 * MLP_IN * 2;  // genes (bf16), then the scores over them
 // waves per workgroup: the MLP variants run 8 (two per SIMD) around one LDS
 // copy of W2; the quadratic scorer runs 4
-template <int SC> constexpr int score_waves() { return SC == 0 ? 4 : 8; }
+enum { SC_QUAD = 0, SC_MLP_BF16 = 1, SC_MLP_FP8 = 2, SC_MLP_F64 = 3 };
+template <int SC> constexpr int score_waves() { return SC == SC_QUAD || SC == SC_MLP_F64 ? 4 : 8; }
 
-enum { SC_QUAD = 0, SC_MLP_BF16 = 1, SC_MLP_FP8 = 2 };
+// LDS of the FP64 scorer (per workgroup of 4 waves): W1 n-major and
+// b1 | b2 | w3 in FP64, then per wave the 64 candidates' genes 0..15 and
+// their scores in FP64.
+constexpr size_t F64_W1_BYTES = (size_t)MLP_H * MLP_IN * 8;
+constexpr size_t F64_VEC_BYTES = 3 * (size_t)MLP_H * 8;
+constexpr size_t F64_WAVE_BYTES = 64 * (size_t)MLP_IN * 8 + 64 * 8;
 
 struct MlpLds {
     const void *w2;        // W2 fragments (bf16 or fp8)
@@ -483,7 +490,80 @@ struct MlpLds {
     const uint32_t *w3h;   // w3 as f16 pairs, [n-tile pair][lane group][4]
     __bf16 *xs;
     float *sc;
+    const double *w1d, *vd;  // FP64 scorer
+    double *xd, *scd;
 };
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// GELU, tanh form, in FP64 (the host reference's definition)
+__device__ __forceinline__ double gelu64(double z) {
+    return 0.5 * z * (1.0 + tanh(0.7978845608028654 * (z + 0.044715 * z * z * z)));
+}
+
+// The MLP in FP64 for the wave's 64 candidates (genes in xd), one 16-candidate
+// tile at a time.  Layer 2 runs on v_mfma_f64_16x16x4f64 as Z2^T = W2^T H1^T:
+// lane l supplies B = H1^T[n][c] with n = 4 ns + l / 16, c = l % 16, which it
+// computes itself (layer 1 and GELU on the VALU), and A = W2[n][16 mt + l % 16]
+// from the packed fragments; D[m][c] lands in register r of lane l with
+// m = 16 mt + 4 r + l / 16, c = l % 16, so each lane finishes layer 3 for 64
+// hidden units of its candidate and two cross-lane adds complete the sum.
+__device__ double mlp_wave_f64(const MlpScreenW &w, const MlpLds &ml, int lane) {
+    const int lr = lane & 15, lq = lane >> 4;
+    const double *b1 = ml.vd, *b2 = ml.vd + MLP_H, *w3 = ml.vd + 2 * MLP_H;
+    for (int ct = 0; ct < 4; ++ct) {
+        const int c = ct * 16 + lr;
+        double x[MLP_IN];
+#pragma unroll
+        for (int i = 0; i < MLP_IN; ++i) x[i] = ml.xd[c * MLP_IN + i];
+        dbl4 acc[16];
+#pragma unroll
+        for (int mt = 0; mt < 16; ++mt) acc[mt] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int ns = 0; ns < MLP_H / 4; ++ns) {
+            const int n = 4 * ns + lq;
+            const double *wr = ml.w1d + n * MLP_IN;
+            double z = 0.0;
+#pragma unroll
+            for (int i = 0; i < MLP_IN; ++i) z = __builtin_fma(x[i], wr[i], z);
+            const double hv = gelu64(z + b1[n]);
+            const double *a2 = w.w2d + (size_t)ns * 16 * 64 + lane;
+#pragma unroll
+            for (int mt = 0; mt < 16; ++mt) acc[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[mt * 64], hv, acc[mt], 0, 0, 0);
+        }
+        double part = 0.0;
+#pragma unroll
+        for (int mt = 0; mt < 16; ++mt) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = 16 * mt + 4 * r + lq;
+                part = __builtin_fma(w3[m], gelu64(acc[mt][r] + b2[m]), part);
+            }
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        if (lq == 0) ml.scd[c] = part + w.b3d;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    return ml.scd[lane];
+}
+
+// FP64 copies of the MLP for mlp_wave_f64, from the FP32 master parameters:
+// W1 n-major, W2 as A fragments, b1 | b2 | w3.
+__global__ void k_pack_f64(const float *__restrict__ p, double *__restrict__ w1d, double *__restrict__ w2d,
+                           double *__restrict__ vd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < MLP_H * MLP_H) {
+        const int l = i & 63, mt = (i >> 6) & 15, ns = i >> 10;
+        w2d[i] = (double)p[P_W2 + (4 * ns + (l >> 4)) * MLP_H + 16 * mt + (l & 15)];
+    }
+    if (i < MLP_H * MLP_IN) {
+        const int n = i / MLP_IN, k = i % MLP_IN;
+        w1d[i] = (double)p[P_W1 + k * MLP_H + n];
+    }
+    if (i < 3 * MLP_H) vd[i] = (double)p[(i < MLP_H ? P_B1 : i < 2 * MLP_H ? P_B2 - MLP_H : P_W3 - 2 * MLP_H) + i];
+}
 
 // One wavefront scores its 64 candidates (genes already in xs) with the MLP.
 // Layer 1 computes H1^T = W1^T X^T on v_mfma_f32_16x16x16_bf16, so each
@@ -736,8 +816,9 @@ template <bool FAST_MUT>
 __device__ __attribute__((noinline)) void emit_mlp_records(Record *rec, uint32_t *rec_count, uint32_t rec_cap,
                                                            const float *el, float sigma, float sig24, bool imm,
                                                            uint64_t s0, uint32_t ia, uint32_t ib, uint32_t gi,
-                                                           float jval, float score, bool pass, uint64_t pm, int lane,
-                                                           uint32_t stream, uint64_t cidx, uint64_t c) {
+                                                           float jval, float score, double score64, bool pass,
+                                                           uint64_t pm, int lane, uint32_t stream, uint64_t cidx,
+                                                           uint64_t c) {
     float g[CS_NTOT];
     if (imm) synth_genes<true, FAST_MUT, 0, CS_NTOT>(g, s0, el, ia, ib, sigma, sig24);
     else synth_genes<false, FAST_MUT, 0, CS_NTOT>(g, s0, el, ia, ib, sigma, sig24);
@@ -759,6 +840,7 @@ __device__ __attribute__((noinline)) void emit_mlp_records(Record *rec, uint32_t
 #pragma unroll
             for (int k = 0; k < CS_NTOT; ++k) o->g[k] = g[k];
             o->_pad = 0;
+            o->score64 = score64;
         }
     }
 }
@@ -769,7 +851,7 @@ template <int IMM, bool LDS_ELITES, bool FAST_MUT, int SC, bool DUMP>
 __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml, const float *mdl,
                                             const float *el, const uint32_t *q, uint32_t head,
                                             uint32_t count, uint64_t bstart, uint64_t first,
-                                            uint64_t seed, uint32_t stream, float bound,
+                                            uint64_t seed, uint32_t stream, float bound, double bound64,
                                             int lane, bool imm_rt = false) {
     const bool imm = IMM == 2 ? imm_rt : IMM == 1;
     const bool act = (uint32_t)lane < count;
@@ -805,7 +887,16 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
         for (int k = 0; k < KH; ++k) g[k] = (k == (int)gi) ? jval : g[k];
     }
     float score;
-    if constexpr (SC != SC_QUAD) {
+    double s64 = 0.0;
+    if constexpr (SC == SC_MLP_F64) {
+#pragma unroll
+        for (int k = 0; k < MLP_IN; ++k) ml.xd[lane * MLP_IN + k] = (double)g[k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        s64 = mlp_wave_f64(a.mlp, ml, lane);
+        score = (float)s64;
+    } else if constexpr (SC != SC_QUAD) {
         bf16x8 x0, x1;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -862,12 +953,13 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
     }
     score = S + r;
     }
-    const bool pass = act && !(score <= bound);
+    if constexpr (SC != SC_MLP_F64) s64 = (double)score;
+    const bool pass = act && (SC == SC_MLP_F64 ? !(s64 <= bound64) : !(score <= bound));
     const uint64_t pm = __ballot(pass);
     if constexpr (KH < CS_NTOT && !DUMP) {
         if (pm)
             emit_mlp_records<FAST_MUT>(a.rec, a.rec_count, a.rec_cap, el, a.sigma, a.sig24, imm, s0, ia, ib, gi, jval,
-                                       score, pass, pm, lane, stream, cidx, c);
+                                       score, s64, pass, pm, lane, stream, cidx, c);
         return;
     }
     if constexpr (KH < CS_NTOT) {
@@ -901,6 +993,7 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
 #pragma unroll
                 for (int k = 0; k < CS_NTOT; ++k) o->g[k] = g[k];
                 o->_pad = 0;
+                o->score64 = s64;
             }
         }
     }
@@ -929,7 +1022,7 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
         el = el_lds;
     }
     MlpLds ml{};
-    if constexpr (SC != SC_QUAD) {
+    if constexpr (SC == SC_MLP_BF16 || SC == SC_MLP_FP8) {
         // after the rings: W2 fragments, b1 | b2 | w3 pairs, then per wave the
         // genes (the scores reuse that buffer once layer 1 has read it).  The
         // FP8 screen carries H1 scaled by the power of two h_scale (exact on
@@ -977,6 +1070,19 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
         ml.w3h = w3h;
         ml.xs = reinterpret_cast<__bf16 *>(wv);
         ml.sc = reinterpret_cast<float *>(wv);
+        __syncthreads();
+    } else if constexpr (SC == SC_MLP_F64) {
+        // after the rings: W1 n-major and b1 | b2 | w3 in FP64, then per wave
+        // the genes and the scores
+        double *w1 = reinterpret_cast<double *>(smem + NW * 2 * QCAP);
+        double *vd = w1 + MLP_H * MLP_IN;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)(MLP_H * MLP_IN); i += blockDim.x) w1[i] = a.mlp.w1d[i];
+        for (uint32_t i = threadIdx.x; i < (uint32_t)(3 * MLP_H); i += blockDim.x) vd[i] = a.mlp.vd[i];
+        double *wv = vd + 3 * MLP_H + (size_t)wave * (F64_WAVE_BYTES / 8);
+        ml.w1d = w1;
+        ml.vd = vd;
+        ml.xd = wv;
+        ml.scd = wv + 64 * MLP_IN;
         __syncthreads();
     }
     uint32_t *qN = qbase + wave * 2 * QCAP;
@@ -1039,13 +1145,13 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
                 enqueue();
                 while (tailN - headN >= 64) {
                     score_batch<0, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, qN, headN, 64, bstart, first, st.seed,
-                                                                   (uint32_t)t, sg.bound, lane);
+                                                                   (uint32_t)t, sg.bound, sg.bound64, lane);
                     headN += 64;
                     screened += 64;
                 }
                 if (tailI - headI >= 64) {
                     score_batch<1, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, qI, headI, 64, bstart, first, st.seed,
-                                                                   (uint32_t)t, sg.bound, lane);
+                                                                   (uint32_t)t, sg.bound, sg.bound64, lane);
                     headI += 64;
                     screened += 64;
                 }
@@ -1053,12 +1159,12 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             }
             if (tailN > headN) {
                 score_batch<0, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, qN, headN, tailN - headN, bstart, first,
-                                                               st.seed, (uint32_t)t, sg.bound, lane);
+                                                               st.seed, (uint32_t)t, sg.bound, sg.bound64, lane);
                 screened += tailN - headN;
             }
             if (tailI > headI) {
                 score_batch<1, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, qI, headI, tailI - headI, bstart, first,
-                                                               st.seed, (uint32_t)t, sg.bound, lane);
+                                                               st.seed, (uint32_t)t, sg.bound, sg.bound64, lane);
                 screened += tailI - headI;
             }
         } else {
@@ -1080,7 +1186,7 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
                 else if (tailI > headI) { im = true; cnt = tailI - headI; }
                 else break;
                 score_batch<2, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, im ? qI : qN, im ? headI : headN, cnt,
-                                                               bstart, first, st.seed, (uint32_t)t, sg.bound, lane,
+                                                               bstart, first, st.seed, (uint32_t)t, sg.bound, sg.bound64, lane,
                                                                im);
                 if (im) headI += cnt;
                 else headN += cnt;
@@ -1210,7 +1316,8 @@ static int launch_ranges(HostState &h, const std::vector<uint64_t> &lo, const st
         segs[t].flat_first = first[t];
         segs[t].b0 = lo[t];
         StreamHeap &hp = h.heaps[t];
-        segs[t].bound = hp.full() ? hp.root() : nan_f();
+        segs[t].bound = hp.full() ? (float)hp.root() : nan_f();  // exact: f32 keys
+        segs[t].bound64 = hp.full() ? hp.root() : (double)nan_f();
         segs[t]._pad = 0;
     }
     const uint64_t nflat = first[T];
@@ -1231,7 +1338,8 @@ static int launch_ranges(HostState &h, const std::vector<uint64_t> &lo, const st
     a.rec_cap = h.rec_cap;
     a.rec_count = (uint32_t *)buf_counters(h, buf);
     a.cand_count = (unsigned long long *)(buf_counters(h, buf) + 8);
-    const unsigned nw = h.sc == SC_QUAD ? score_waves<SC_QUAD>() : score_waves<SC_MLP_BF16>();
+    const unsigned nw = h.sc == SC_QUAD ? score_waves<SC_QUAD>()
+                        : h.sc == SC_MLP_F64 ? score_waves<SC_MLP_F64>() : score_waves<SC_MLP_BF16>();
     uint64_t grid = (nflat + nw - 1) / nw;
     if (grid > (1u << 20)) grid = 1u << 20;
     HIPCHK(hipEventRecord(buf_ev_start(ctx, buf), ctx->stream));
@@ -1249,7 +1357,11 @@ static int launch_ranges(HostState &h, const std::vector<uint64_t> &lo, const st
     if (a.dump_genes) {  // evaluation runs: every candidate's genes and score
         if (h.sc == SC_MLP_BF16) go(k_score<false, false, SC_MLP_BF16, true>);
         else if (h.sc == SC_MLP_FP8) go(k_score<false, false, SC_MLP_FP8, true>);
+        else if (h.sc == SC_MLP_F64) go(k_score<false, false, SC_MLP_F64, true>);
         else go(k_score<false, false, SC_QUAD, true>);
+    } else if (h.sc == SC_MLP_F64) {
+        if (h.fast_mut) go(k_score<false, true, SC_MLP_F64>);
+        else go(k_score<false, false, SC_MLP_F64>);
     } else if (h.sc == SC_MLP_BF16) {
         if (h.fast_mut) go(k_score<false, true, SC_MLP_BF16>);
         else go(k_score<false, false, SC_MLP_BF16>);
@@ -1335,7 +1447,9 @@ static int finish_ranges(HostState &h, const std::vector<uint64_t> &lo, const st
         for (uint32_t k = 0; k < cnt; ++k) {
             const Record &r = ctx->host_rec[(uint32_t)order[k].first];
             cs_survivor s;
-            s.score = r.score;
+            // the FP64 scorer ranks by its FP64 score (kept as the key)
+            const double key = h.sc == SC_MLP_F64 ? r.score64 : (double)r.score;
+            s.score = h.sc == SC_MLP_F64 ? (float)r.score64 : r.score;
             std::memcpy(s.g, r.g, sizeof s.g);
             s.pa = h.elites[r.ia].id;
             s.pb = h.elites[r.ib].id;
@@ -1343,7 +1457,7 @@ static int finish_ranges(HostState &h, const std::vector<uint64_t> &lo, const st
             s.stream = r.stream;
             s.index = r.index;
             s.start = r.start;
-            h.heaps[r.stream].offer(s);
+            h.heaps[r.stream].offer(s, key);
         }
     }
     h.ms_replay += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1523,7 +1637,7 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     if (ne < 2) return fail(CS_E_NEEDTWO, "NeedTwoElites");
     if (cfg->keep == 0) return fail(CS_E_KEEP0, "keep must be >= 1 (keep=0 is undefined in the reference)");
     const int sc = opts ? opts->scorer : SC_QUAD;
-    if (sc < SC_QUAD || sc > SC_MLP_FP8) return fail(CS_E_BADARGS, "unknown scorer");
+    if (sc < SC_QUAD || sc > SC_MLP_F64) return fail(CS_E_BADARGS, "unknown scorer");
     if (sc != SC_QUAD && (!opts->mlp || opts->mlp->ctx != ctx))
         return fail(CS_E_BADARGS, "the MLP scorers need an MLP created on this context");
     if (sc == SC_QUAD && !model) return fail(CS_E_BADARGS, "the quadratic scorer needs a model");
@@ -1643,6 +1757,18 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         // FP8 layer 2: the descale also applies the GELU input scale q
         mw.descale = GELU_Q / (ml->h_scale * ml->w_scale);
         mw.q1 = sc == SC_MLP_FP8 ? GELU_Q / ml->h_scale : GELU_Q;
+        if (sc == SC_MLP_F64) {
+            // FP64 copies of the FP32 master parameters, repacked per call
+            const size_t nd = (size_t)MLP_H * MLP_H + (size_t)MLP_H * MLP_IN + 3 * MLP_H;
+            if ((rc = ensure(ml->f64pack, nd * 8))) return rc;
+            double *w2d = (double *)ml->f64pack.p, *w1d = w2d + MLP_H * MLP_H, *vd = w1d + MLP_H * MLP_IN;
+            hipLaunchKernelGGL(k_pack_f64, dim3(MLP_H * MLP_H / 256), dim3(256), 0, ctx->stream, pp, w1d, w2d, vd);
+            HIPCHK(hipGetLastError());
+            mw.w1d = w1d;
+            mw.w2d = w2d;
+            mw.vd = vd;
+            mw.b3d = (double)b3;
+        }
     }
     ScoreArgs &a = h.sa;
     a.streams = (const StreamInfo *)h.B->streams.p;
@@ -1673,6 +1799,9 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     h.lds_elites = sc == SC_QUAD && el_bytes <= 48 * 1024 && h.sa.dump_genes == nullptr;
     if (sc == SC_QUAD) {
         h.smem = score_waves<SC_QUAD>() * 2 * QCAP * 4 + (h.lds_elites ? LDS_MODEL_BYTES + el_bytes : 0);
+    } else if (sc == SC_MLP_F64) {
+        const size_t nw = score_waves<SC_MLP_F64>();
+        h.smem = nw * 2 * QCAP * 4 + F64_W1_BYTES + F64_VEC_BYTES + nw * F64_WAVE_BYTES;
     } else {
         const size_t nw = score_waves<SC_MLP_BF16>();
         h.smem = nw * 2 * QCAP * 4 + (sc == SC_MLP_FP8 ? MLP_W2_FP8_BYTES : MLP_W2_BF16_BYTES) + MLP_VEC_BYTES +
@@ -1715,12 +1844,14 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
     }
     // ---- 4. merge heaps t-major, stable sort by score descending
     auto tm0 = std::chrono::steady_clock::now();
-    std::vector<cs_survivor> all;
+    std::vector<HeapItem> all;
     for (int t = 0; t < T; ++t) all.insert(all.end(), h.heaps[t].slots.begin(), h.heaps[t].slots.end());
-    std::stable_sort(all.begin(), all.end(), [](const cs_survivor &x, const cs_survivor &y) { return x.score > y.score; });
+    std::stable_sort(all.begin(), all.end(), [](const HeapItem &x, const HeapItem &y) { return x.key > y.key; });
     uint64_t out_n = std::min<uint64_t>(all.size(), cfg->keep);
     if (out_n > cap) return fail(CS_E_CAP, "survivor buffer too small");
-    std::memcpy(out, all.data(), out_n * sizeof(cs_survivor));
+    for (uint64_t i = 0; i < out_n; ++i) out[i] = all[i].s;
+    if (opts && opts->scores64)
+        for (uint64_t i = 0; i < out_n; ++i) opts->scores64[i] = all[i].key;
     *n_out = out_n;
     *screens = lambda;
     h.ms_replay += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tm0).count();

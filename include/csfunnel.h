@@ -196,6 +196,9 @@ int cs_mlp_train_step(cs_mlp *mlp, const float *genes, int ldg, const float *y, 
 #define CS_SCORER_QUAD 0     /* src/screen.zig:97-112, exact */
 #define CS_SCORER_MLP_BF16 1 /* learned surrogate, BF16 MFMA */
 #define CS_SCORER_MLP_FP8 2  /* learned surrogate, FP8 MFMA for layer 2 */
+#define CS_SCORER_MLP_F64 3  /* learned surrogate in FP64 (FP64 MFMA layer 2): the
+                                reference the reduced-precision screens' top-k
+                                recall is measured against */
 
 typedef struct cs_screen_opts {
     int scorer;        /* CS_SCORER_* */
@@ -203,6 +206,8 @@ typedef struct cs_screen_opts {
     cs_mlp *mlp;       /* required by the MLP scorers */
     float *dump_genes;  /* optional DEVICE [lambda][29]: every candidate's genes, stream order */
     float *dump_scores; /* DEVICE [lambda], required with dump_genes: every candidate's score */
+    double *scores64;   /* optional HOST [cap]: each survivor's ranking key (the FP64
+                           score for CS_SCORER_MLP_F64, the f32 score otherwise) */
 } cs_screen_opts;
 
 /* cs_screen with a choice of scorer and optional dumps; the synthesis,

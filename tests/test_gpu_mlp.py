@@ -205,7 +205,12 @@ def test_mlp_screen_bf16_recall(dev, trained):
     print("case MLP screen bf16 lam=2^22 keep=64", flush=True)
     recall, dmax, _, _ = _screen_recall(dev, m, 1, 1 << 22, 64)
     print(f"BF16 MLP screen: top-64 recall vs FP64 = {recall:.4f}, max |score - fp64| = {dmax:.3e}", flush=True)
-    assert recall >= 0.9
+    # Raw BF16 order can miss a near-tie at the cut: the FP64 scores of the
+    # 64th and 65th candidates differ by ~1e-2 while BF16 rounding of the
+    # genes alone moves scores by up to ~0.5 (CPU emulation of the rounding
+    # steps), so at most one miss in 64 is allowed raw; the funnel's FP64
+    # re-rank of the BF16 top 256 below must reach 0.99.
+    assert recall >= 0.98
     frec = _funnel_recall(dev, m, 1, 1 << 22, 64)
     print(f"BF16 screen + FP64 re-rank of the top 256: top-64 recall vs FP64 = {frec:.4f}", flush=True)
     assert frec >= 0.99
@@ -284,3 +289,38 @@ def test_mlp_screen_survivor_genomes(dev, trained, scorer):
     per = (1 << 21) // 4
     rows = a["stream"].astype(np.int64) * per + a["index"].astype(np.int64)
     assert np.array_equal(a["g"], dg.cpu().numpy()[rows])
+
+
+def test_f64_scorer_is_the_exact_fp64_top_k(dev, trained):
+    """The FP64 scorer (FP64 MFMA layer 2) keeps the exact FP64 top-k of every
+    candidate: its scores agree with the host FP64 reference within 1e-12,
+    its survivors are the top-k of the host FP64 scores of every dumped
+    genome, and the plain screen returns the same survivors as the dumping
+    one."""
+    from csfunnel import screen as scr
+    from csfunnel.mlp import SCORER_MLP_F64, forward_f64, screen_ex, screen_f64
+    m = trained[0]
+    g = os.path.join(REPO, "tests", "golden")
+    el = scr.load_elites(os.path.join(g, "c1_elites.txt"))
+    lam, keep = 1 << 20, 64
+    cfg = scr.Cfg(lam, keep, 5, 4, float(np.float32(0.12)), float(np.float32(0.01)), float(np.float32(0.01)))
+    print("case FP64 scorer lam=2^20 keep=64", flush=True)
+    keys = np.zeros(keep, np.float64)
+    a, n, dg, ds = screen_ex(dev, None, el, cfg, SCORER_MLP_F64, m, dump=True, keys=keys)
+    b, s64, n2 = screen_f64(dev, el, cfg, m)
+    assert n == n2 == lam and len(a) == len(b) == keep
+    assert np.array_equal(a, b) and np.array_equal(keys, s64)
+    assert np.all(np.diff(s64) <= 0)
+    ref = forward_f64(m.params, b["g"])
+    err = np.abs(s64 - ref) / (1.0 + np.abs(ref))
+    print(f"FP64 scorer: max relative |device - host| = {err.max():.3e}", flush=True)
+    assert err.max() <= 1e-12
+    assert np.array_equal(b["score"], s64.astype(np.float32))
+    allref = forward_f64(m.params, dg[:, :16].cpu().numpy())
+    order = np.argsort(-allref, kind="stable")
+    per = lam // 4
+    rows = b["stream"].astype(np.int64) * per + b["index"].astype(np.int64)
+    assert np.array_equal(b["g"], dg.cpu().numpy()[rows])
+    gap = allref[order[keep - 1]] - allref[order[keep]]
+    print(f"host top-{keep} cut gap {gap:.3e}", flush=True)
+    assert set(rows.tolist()) == set(order[:keep].tolist())
