@@ -155,19 +155,21 @@ def mlp_recall(dev, elites, scorer, mlp):
     return raw, len(top & set(_rows(fun, RECALL_LAMBDA))) / 64
 
 
-def production_recall(dev, elites, mlp, scorers):
+def production_recall(dev, elites, mlp, scorers, ref=None):
     """Top-64 recall at the production size: the exact FP64 top 64 of the MLP
     over all 2.65e9 candidates of generation 0 (the device FP64 scorer, no
     dump) against each reduced-precision screen's top 64, raw and after the
     funnel's FP64 re-rank of its top 256, identified by (stream, index)."""
     from csfunnel.mlp import screen_ex, screen_f64, screen_funnel
     cfg = _cfg(LAMBDA, 64, 0)
-    t0 = time.perf_counter()
-    ref, s64, n = screen_f64(dev, elites, cfg, mlp)
-    f64_s = time.perf_counter() - t0
-    assert n == LAMBDA and len(ref) == 64
+    if ref is None:
+        t0 = time.perf_counter()
+        surv, s64, n = screen_f64(dev, elites, cfg, mlp)
+        assert n == LAMBDA and len(surv) == 64
+        ref = (surv, s64, time.perf_counter() - t0)
+    ref_surv, s64, f64_s = ref
     key = lambda a: set(zip(a["stream"].tolist(), a["index"].tolist()))  # noqa: E731
-    top = key(ref)
+    top = key(ref_surv)
     out = {"lambda": LAMBDA, "k": 64, "fp64_screen_s": round(f64_s, 3),
            "fp64_cut_gap": float(s64[62] - s64[63]), "reference": "exact FP64 top 64 (device FP64 scorer)"}
     for name, sc in scorers:
@@ -177,7 +179,7 @@ def production_recall(dev, elites, mlp, scorers):
         out[name + "_after_rerank"] = len(top & key(fun)) / 64
         print(f"recall at 2.65e9, {name}: {out[name]:.4f} raw, {out[name + '_after_rerank']:.4f} after re-rank "
               f"(FP64 reference screen {f64_s:.1f} s)", file=sys.stderr, flush=True)
-    return out
+    return out, ref
 
 
 def _cfg(lam, keep, gen):
@@ -281,8 +283,10 @@ def main():
                     help="candidates of the 4-thread CPU leg (the all-threads leg scales it by threads / 4)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mlp", action="store_true", help="headline line without the learned-surrogate funnel")
-    ap.add_argument("--funnel-gens", type=int, default=2,
+    ap.add_argument("--funnel-gens", type=int, default=4,
                     help="generations of the quadratic-vs-MLP funnel re-scored by the codec label source (0: skip)")
+    ap.add_argument("--refit-gens", type=int, default=6,
+                    help="generations of the same funnel with every arm refit on its labelled survivors (0: skip)")
     ap.add_argument("--scorer", choices=["quad", "mlp-bf16", "mlp-fp8"], default="quad",
                     help="quad: the reference's quadratic surface (headline); mlp-*: the learned surrogate")
     args = ap.parse_args()
@@ -304,7 +308,7 @@ def main():
         else:
             train_info.pop("genes")
         dt, stats = run_generations(dev, model, elites, scorer, mlp, args.steps, args.warmup, 64)
-        prod = production_recall(dev, elites, mlp, [(args.scorer, scorer)])
+        prod, _ = production_recall(dev, elites, mlp, [(args.scorer, scorer)])
         raw, rr = mlp_recall(dev, elites, scorer, mlp)
         rates = scr.valu_rates(0)
         dev.close()
@@ -332,7 +336,7 @@ def main():
         mlp, wq, train_info = train_surrogate(dev)
         genes = train_info.pop("genes")
         mdt, mstats = run_generations(dev, model, elites, 1, mlp, 2, 1, 64)
-        prod = production_recall(dev, elites, mlp, [("mlp_bf16", 1)])
+        prod, f64ref = production_recall(dev, elites, mlp, [("mlp_bf16", 1)])
         raw, rr = mlp_recall(dev, elites, 1, mlp)
         recall.update({"mlp_bf16": raw, "mlp_bf16_after_rerank": rr, "k_mlp": 64})
         mlp_out = mlp_line(1, 2, mdt, mstats, train_info, rates)
@@ -341,7 +345,7 @@ def main():
         fdt, fstats = run_generations(dev, model, elites, 2, mlp, 2, 1, 64)
         fraw, frr = mlp_recall(dev, elites, 2, mlp)
         recall.update({"mlp_fp8": fraw, "mlp_fp8_after_rerank": frr})
-        fprod = production_recall(dev, elites, mlp, [("mlp_fp8", 2)])
+        fprod, _ = production_recall(dev, elites, mlp, [("mlp_fp8", 2)], ref=f64ref)
         prod.update({k: v for k, v in fprod.items() if k.startswith("mlp_fp8")})
         recall["mlp_production"] = prod
         fp8_line = mlp_line(2, 2, fdt, fstats, train_info, rates)
@@ -350,10 +354,19 @@ def main():
             from csfunnel import codec, surrogate
             print("funnel: quadratic vs MLP screen, survivors re-scored by the codec label source", file=sys.stderr,
                   flush=True)
+            ev = codec.Evaluator()
             funnel = surrogate.funnel_compare(
-                dev, elites, mlp, wq, codec.Evaluator(), gens=args.funnel_gens,
+                dev, elites, mlp, wq, ev, gens=args.funnel_gens,
                 progress=lambda r: print("funnel gen %d: quad %.3f mlp %.3f ensemble %.3f" % (
                     r["gen"], r["quad_mean"], r["mlp_mean"], r["ensemble_mean"]), file=sys.stderr, flush=True))
+            funnel["learned_funnel"] = "ensemble: BF16 MLP screen, top 1024 re-ranked by the mean FP64 MLP and quadratic prediction"
+            if args.refit_gens > 0:
+                from csfunnel import make_labels
+                lg, ly, lntr, _ = make_labels.load()
+                funnel["refit_each_generation"] = surrogate.funnel_compare(
+                    dev, elites, mlp, wq, ev, gens=args.refit_gens, train=(lg[:lntr], ly[:lntr]),
+                    progress=lambda r: print("refit funnel gen %d: quad %.3f mlp %.3f ensemble %.3f" % (
+                        r["gen"], r["quad_mean"], r["mlp_mean"], r["ensemble_mean"]), file=sys.stderr, flush=True))
     dev.close()
     ms_score = sum(s["ms_score"] for s in stats)
     ms_map = sum(s["ms_map"] for s in stats)

@@ -195,13 +195,19 @@ def forward_f64(params: np.ndarray, genes: np.ndarray) -> np.ndarray:
     return gelu(gelu(x @ W1 + b1) @ W2 + b2) @ w3 + float(b3)
 
 
-def screen_funnel(dev: Screen, elites, cfg: Cfg, scorer: int, mlp: MLP, rerank: int = 4):
+def screen_funnel(dev: Screen, elites, cfg: Cfg, scorer: int, mlp: MLP, rerank: int = 4,
+                  quad_w: Optional[np.ndarray] = None):
     """The full funnel: the device screen keeps rerank * keep candidates by
     the BF16 / FP8 score, the host re-scores those in FP64 and returns the top
-    `keep` (scores replaced by the FP64 values, FP64 order)."""
+    `keep` (scores replaced by the FP64 values, FP64 order).  With quad_w (a
+    153-term surface over genes 0..15 predicting the same label) the re-rank
+    score is the mean of the two FP64 predictions."""
     wide = Cfg(cfg.lambda_, cfg.keep * rerank, cfg.seed, cfg.threads, cfg.sigma, cfg.jump_p, cfg.imm_p)
     surv, screens, _, _ = screen_ex(dev, None, elites, wide, scorer, mlp)
     s64 = forward_f64(mlp.params, surv["g"])
+    if quad_w is not None:
+        from .surrogate import quad_predict
+        s64 = 0.5 * (s64 + quad_predict(quad_w, np.asarray(surv["g"], dtype=np.float32)[:, :16]))
     order = np.argsort(-s64, kind="stable")[: cfg.keep]
     out = surv[order].copy()
     out["score"] = s64[order].astype(np.float32)

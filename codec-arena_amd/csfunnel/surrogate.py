@@ -96,7 +96,9 @@ def _next_elites(surv, gen: int):
 def funnel_compare(dev: Screen, elites, m: MLP, w: np.ndarray, evaluator, gens: int = 2,
                    lam: int = 2_650_000_000, keep: int = 64, threads: int = 4, scorer: int = SCORER_MLP_BF16,
                    sigma0: float = 0.12, jump: float = 0.01, imm: float = 0.01, pool: int = 1024,
-                   progress: Optional[Callable[[Dict[str, float]], None]] = None) -> Dict[str, object]:
+                   progress: Optional[Callable[[Dict[str, float]], None]] = None,
+                   train: Optional[tuple] = None, replicate: int = 4,
+                   mlp_epochs: int = 300) -> Dict[str, object]:
     """Three screens from the same elites for `gens` chained generations
     (survivors become that screen's next elites, genes 16..28 reset to the
     legacy values); each generation's `keep` survivors are re-scored by the
@@ -109,10 +111,20 @@ def funnel_compare(dev: Screen, elites, m: MLP, w: np.ndarray, evaluator, gens: 
     Returns per-generation oracle statistics and timings, and per arm the
     mean over generations of the survivors' mean label, the best label seen
     and the last generation's mean.  `progress` is called with each
-    generation's record as it completes."""
-    qm = quad_model(w)
+    generation's record as it completes.
+
+    train = (genes, labels): refit each arm's surrogates after every
+    generation on these pairs plus every codec-labelled survivor of that arm
+    so far (each repeated `replicate` times), as the reference's loop refits
+    its surface on oracle rows (brain/evolve.py:474-478): the quadratic
+    surface by the FP64 ridge, the MLP by `mlp_epochs` of AdamW from a fresh
+    initialisation.  Without it the surrogates stay fixed."""
     arms = ("quad", "mlp", "ensemble")
     el = {a: elites for a in arms}
+    qw = {a: w for a in arms}       # each arm's quadratic weights
+    mm = {a: m for a in arms}       # each arm's MLP
+    seen = {a: ([], []) for a in arms}  # each arm's labelled survivors
+    owned: List[MLP] = []
     out: List[Dict[str, float]] = []
     base = evaluator.labels(_legacy_rows({"g": np.array([list(e.g)[:16] for e in elites])}))
     for gen in range(gens):
@@ -122,17 +134,12 @@ def funnel_compare(dev: Screen, elites, m: MLP, w: np.ndarray, evaluator, gens: 
         for name in arms:
             t0 = time.perf_counter()
             if name == "quad":
-                surv, n = dev.run(qm, el[name], cfg)
+                surv, n = dev.run(quad_model(qw[name]), el[name], cfg)
             elif name == "mlp":
-                surv, n = screen_funnel(dev, el[name], cfg, scorer, m, rerank=4)
+                surv, n = screen_funnel(dev, el[name], cfg, scorer, mm[name], rerank=4)
             else:
-                wcfg = Cfg(lam, pool, gen, threads, sg, float(np.float32(jump)), float(np.float32(imm)))
-                wide, n, _, _ = screen_ex(dev, None, el[name], wcfg, scorer, m)
-                g16 = _legacy_rows(wide)
-                ens = 0.5 * (forward_f64(m.params, g16) + quad_predict(w, g16))
-                order = np.argsort(-ens, kind="stable")[:keep]
-                surv = wide[order].copy()
-                surv["score"] = ens[order].astype(np.float32)
+                surv, n = screen_funnel(dev, el[name], cfg, scorer, mm[name], rerank=pool // keep,
+                                        quad_w=qw[name])
             rec[f"{name}_screen_s"] = time.perf_counter() - t0
             assert n == lam and len(surv) == keep
             t0 = time.perf_counter()
@@ -142,9 +149,24 @@ def funnel_compare(dev: Screen, elites, m: MLP, w: np.ndarray, evaluator, gens: 
             rec[f"{name}_max"] = float(lab.max())
             rec[f"{name}_top8_mean"] = float(np.sort(lab)[-8:].mean())
             el[name] = _next_elites(surv, gen)
+            if train is not None and gen + 1 < gens:
+                t0 = time.perf_counter()
+                seen[name][0].append(_legacy_rows(surv))
+                seen[name][1].append(np.asarray(lab, dtype=np.float64))
+                G = np.concatenate([np.asarray(train[0], np.float32)[:, :16]] + seen[name][0] * replicate)
+                Y = np.concatenate([np.asarray(train[1], np.float64)] + seen[name][1] * replicate)
+                if name in ("quad", "ensemble"):
+                    qw[name] = fit_quadratic(dev, G, Y)
+                if name in ("mlp", "ensemble"):
+                    nm, _ = train_mlp(dev, G, Y, epochs=mlp_epochs)
+                    owned.append(nm)
+                    mm[name] = nm
+                rec[f"{name}_refit_s"] = time.perf_counter() - t0
         out.append(rec)
         if progress is not None:
             progress(rec)
+    for nm in owned:
+        nm.close()
     summary = {a: {"mean_of_means": float(np.mean([r[f"{a}_mean"] for r in out])),
                    "best": float(max(r[f"{a}_max"] for r in out)),
                    "last_mean": out[-1][f"{a}_mean"]} for a in arms}
