@@ -54,3 +54,16 @@ def test_mfma_fraction_against_the_layer2_instruction(scorer, ins, flop):
     assert m["probe_instruction"] == ins
     assert m["frac_of_probe"] == pytest.approx(1.0, rel=1e-12)
     assert m["ns_4pk"] == pytest.approx(4 * 1.8)
+
+
+def test_profiled_constants_describe_the_built_source():
+    """roofline.traffic and the VALU instruction count are copied from the
+    rocprof summary in profiles/; they hold only for the kernel source that
+    was profiled, so a change to that source must come with a new profile
+    (and the new hash here and in the summary)."""
+    assert bench.source_sha(bench.PROFILED_SOURCE) == bench.PROFILED_SHA16
+    with open(os.path.join(REPO, "profiles", "screen_walk_offsets_2p65e9.md")) as f:
+        summary = f.read()
+    assert f"sha256 prefix `{bench.PROFILED_SHA16}`" in summary
+    assert f"| `k_score` | SQ_INSTS_VALU | 7.699e10 | {bench.VALU_LANE_INSTR}." in summary
+    assert f"{bench.SCORE_TRAFFIC_PER_LAUNCH / 1e6:.2f} MB per\nlaunch" in summary
