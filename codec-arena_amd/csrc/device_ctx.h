@@ -57,7 +57,7 @@ struct cs_ctx {
     cs_screen_stats stats{};
     uint32_t rec_cap = 1u << 20;
     bool force_maps = false;  // testing hook: always use the transfer maps
-    int walk_seg_log = 24;    // speculative walk: 2^walk_seg_log draws per segment
+    int walk_seg_log = 0;     // speculative walk: 0 = balanced segments, else 2^walk_seg_log draws each
     int n_cu = 256;
     int n_mlp = 0;  // live cs_mlp handles bound to this context
     void *scratch = nullptr;  // this device's copy of the static scratch array

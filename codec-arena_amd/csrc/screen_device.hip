@@ -334,10 +334,15 @@ __global__ __launch_bounds__(64) void k_down(const uint64_t *__restrict__ in,
 // the true chain iff its first start >= segment start (x_s) equals the first
 // start >= that point found by segment s-1 (y_{s-1}); segment 0 starts at the
 // true stream start.  Unverified segments are re-walked from y_{s-1}.
-// Segment length: 2^seg_log draws (2^(seg_log - 16) blocks), a context
-// setting (cs_ctx::walk_seg_log; CSFUNNEL_WALK_SEG_LOG in [18, 30] at
-// context creation overrides the default, for measurement).
-constexpr int SEG_LOG_DEFAULT = 24;
+// Segment length: seg_blocks blocks.  By default the generation's blocks are
+// cut into as many equal segments as the device keeps k_walk waves resident
+// at once (eight per SIMD), so that every segment runs in one round with no
+// tail of late waves, but never into segments shorter than MIN_SEG_BLOCKS
+// blocks (2^24 draws), below which the burn-in dominates.  For measurement,
+// CSFUNNEL_WALK_SEG_LOG in [18, 30] at context creation fixes 2^seg_log
+// draws per segment instead.
+constexpr uint64_t MIN_SEG_BLOCKS = 256;
+constexpr uint64_t WALK_WAVES_PER_SIMD = 8;
 constexpr uint64_t BURN = 1ull << 22;  // ~14x the mean merge distance
 
 struct SegOut {
@@ -407,21 +412,20 @@ __global__ __launch_bounds__(64) void k_walk(const StreamInfo *__restrict__ stre
                                              const uint64_t *__restrict__ seg_first, int T, uint64_t nseg,
                                              uint32_t immT, uint32_t jumpT, uint64_t *__restrict__ entries,
                                              SegOut *__restrict__ segout, int64_t fix_flat, uint64_t fix_start,
-                                             int seg_log) {
+                                             uint64_t seg_blocks) {
     const int lane = threadIdx.x;
     for (uint64_t f = blockIdx.x; f < nseg; f += gridDim.x) {
         const uint64_t ff = fix_flat >= 0 ? (uint64_t)fix_flat : f;
         const int t = find_seg(seg_first, T, ff);
         const uint64_t s = ff - seg_first[t];
         const StreamInfo st = streams[t];
-        const uint64_t seg_start = s << seg_log;
-        const uint64_t seg_end = min(seg_start + (1ull << seg_log), st.nblocks << BLOCK_LOG);
+        const uint64_t b0 = s * seg_blocks;
+        const uint64_t seg_start = b0 << BLOCK_LOG;
+        const uint64_t seg_end = min(b0 + seg_blocks, st.nblocks) << BLOCK_LOG;
         uint64_t c;
         if (fix_flat >= 0) c = fix_start;
         else c = seg_start > BURN ? seg_start - BURN : 0;  // 0: the true stream start
-        walk_segment(st.seed, c, seg_start, seg_end, immT, jumpT,
-                     entries + st.blk_first + (s << (seg_log - BLOCK_LOG)),
-                     segout + ff, lane);
+        walk_segment(st.seed, c, seg_start, seg_end, immT, jumpT, entries + st.blk_first + b0, segout + ff, lane);
         if (fix_flat >= 0) break;
     }
 }
@@ -429,11 +433,11 @@ __global__ __launch_bounds__(64) void k_walk(const StreamInfo *__restrict__ stre
 // entries[b] += segment base index (per stream prefix of segment totals)
 __global__ void k_rebase(const StreamInfo *__restrict__ streams, const uint64_t *__restrict__ blk_first, int T,
                          uint64_t nblk, const uint64_t *__restrict__ seg_first,
-                         const uint64_t *__restrict__ seg_base, uint64_t *__restrict__ entries, int seg_blocks_log) {
+                         const uint64_t *__restrict__ seg_base, uint64_t *__restrict__ entries, uint64_t seg_blocks) {
     for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nblk; f += (uint64_t)gridDim.x * blockDim.x) {
         const int t = find_seg(blk_first, T, f);
         const uint64_t b = f - blk_first[t];
-        entries[f] += seg_base[seg_first[t] + (b >> seg_blocks_log)] << 8;
+        entries[f] += seg_base[seg_first[t] + b / seg_blocks] << 8;
     }
 }
 
@@ -1559,11 +1563,18 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
     const int T = h.T;
     int rc;
     *ok = false;
-    const int seg_log = ctx->walk_seg_log, seg_blocks_log = seg_log - BLOCK_LOG;
+    // ceil(nb / seg_blocks) <= nb / seg_blocks + 1 per stream, so seg_blocks
+    // >= nblk / (resident - T) keeps the segment count within the resident
+    // waves
+    const uint64_t resident = WALK_WAVES_PER_SIMD * 4 * (uint64_t)ctx->n_cu;
+    uint64_t seg_blocks = MIN_SEG_BLOCKS;
+    if (ctx->walk_seg_log) seg_blocks = 1ull << (ctx->walk_seg_log - BLOCK_LOG);
+    else if (resident > (uint64_t)T)
+        seg_blocks = std::max(MIN_SEG_BLOCKS, (nblk + resident - T - 1) / (resident - T));
     std::vector<uint64_t> seg_first(T + 1, 0);
     for (int t = 0; t < T; ++t) {
         const uint64_t nb = h.si[t].nblocks;
-        seg_first[t + 1] = seg_first[t] + ((nb + (1ull << seg_blocks_log) - 1) >> seg_blocks_log);
+        seg_first[t + 1] = seg_first[t] + (nb + seg_blocks - 1) / seg_blocks;
     }
     const uint64_t nseg = seg_first[T];
     if ((rc = acquire(h, h.B->seg_first_buf, sizeof(uint64_t) * (T + 1)))) return rc;
@@ -1575,7 +1586,7 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
         unsigned grid = (unsigned)std::min<uint64_t>(nseg, 1u << 20);
         hipLaunchKernelGGL(k_walk, dim3(grid), dim3(64), 0, ctx->stream, (const StreamInfo *)h.B->streams.p,
                            (const uint64_t *)h.B->seg_first_buf.p, T, nseg, immT, jumpT, (uint64_t *)h.B->entries.p,
-                           (SegOut *)h.B->segout.p, (int64_t)-1, (uint64_t)0, seg_log);
+                           (SegOut *)h.B->segout.p, (int64_t)-1, (uint64_t)0, seg_blocks);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -1594,7 +1605,7 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
             hipLaunchKernelGGL(k_walk, dim3(1), dim3(64), 0, ctx->stream, (const StreamInfo *)h.B->streams.p,
                                (const uint64_t *)h.B->seg_first_buf.p, T, (uint64_t)1, immT, jumpT,
                                (uint64_t *)h.B->entries.p, (SegOut *)h.B->segout.p, (int64_t)f, so[f - 1].y,
-                               seg_log);
+                               seg_blocks);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(&so[f], (SegOut *)h.B->segout.p + f, sizeof(SegOut), hipMemcpyDeviceToHost,
                                   ctx->stream));
@@ -1617,7 +1628,7 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
         unsigned grid = (unsigned)std::min<uint64_t>((nblk + 255) / 256, 1u << 16);
         hipLaunchKernelGGL(k_rebase, dim3(grid), dim3(256), 0, ctx->stream, (const StreamInfo *)h.B->streams.p,
                            (const uint64_t *)h.B->blk_first.p, T, nblk, (const uint64_t *)h.B->seg_first_buf.p,
-                           (const uint64_t *)h.B->seg_base.p, (uint64_t *)h.B->entries.p, seg_blocks_log);
+                           (const uint64_t *)h.B->seg_base.p, (uint64_t *)h.B->entries.p, seg_blocks);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
@@ -1891,7 +1902,7 @@ int cs_ctx_create(int device, cs_ctx **out) {
     cs_ctx *c = new cs_ctx();
     c->device = device;
     c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-    c->walk_seg_log = SEG_LOG_DEFAULT;
+    c->walk_seg_log = 0;  // balanced segments
     if (const char *e = std::getenv("CSFUNNEL_WALK_SEG_LOG")) {
         const int v = std::atoi(e);
         if (v >= 18 && v <= 30) c->walk_seg_log = v;
