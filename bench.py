@@ -340,7 +340,7 @@ def main():
         raw, rr = mlp_recall(dev, elites, 1, mlp)
         recall.update({"mlp_bf16": raw, "mlp_bf16_after_rerank": rr, "k_mlp": 64})
         mlp_out = mlp_line(1, 2, mdt, mstats, train_info, rates)
-        # the same network with layer 2 in FP8 (per-tensor power-of-two scales)
+        # the same network with layer 2 in FP8 (E8M0 block scales for W2, a power of two for H1)
         mlp.calibrate_fp8(genes)
         fdt, fstats = run_generations(dev, model, elites, 2, mlp, 2, 1, 64)
         fraw, frr = mlp_recall(dev, elites, 2, mlp)

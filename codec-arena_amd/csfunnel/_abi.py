@@ -82,7 +82,7 @@ SIGNATURES = {
     "cs_mlp_destroy": (C.c_int, [C.c_void_p]),
     "cs_mlp_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "cs_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
-    "cs_mlp_set_fp8_scales": (C.c_int, [C.c_void_p, C.c_float, C.c_float]),
+    "cs_mlp_set_fp8_scale": (C.c_int, [C.c_void_p, C.c_float]),
     "cs_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_void_p, C.POINTER(C.c_float)]),
     "cs_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_float),
                               C.POINTER(C.c_float)]),

@@ -108,17 +108,15 @@ class MLP:
                                       g.shape[0], C.c_float(lr), C.c_float(wd), C.byref(loss)))
         return loss.value
 
-    def calibrate_fp8(self, genes, headroom: float = 224.0) -> Tuple[float, float]:
-        """Per-tensor E4M3 scales: the largest powers of two that keep max |H1|
-        and max |W2| within `headroom` (half of E4M3's 448, leaving room for
-        candidates beyond the sample)."""
+    def calibrate_fp8(self, genes, headroom: float = 224.0) -> float:
+        """The E4M3 scale of H1, layer 2's activations: the largest power of
+        two that keeps max |H1| over `genes` within `headroom` (half of
+        E4M3's 448, leaving room for candidates beyond the sample).  W2 gets
+        per-block E8M0 scales from the library."""
         hmax = max(self.h1_max(genes), 1e-12)
-        W2 = unpack(self.params)[2]
-        wmax = max(float(np.abs(W2).max()), 1e-12)
-        p2 = lambda x: float(2.0 ** min(15, max(-15, int(np.floor(np.log2(x))))))
-        hs, ws = p2(headroom / hmax), p2(headroom / wmax)
-        check(lib().cs_mlp_set_fp8_scales(self._h, C.c_float(hs), C.c_float(ws)))
-        return hs, ws
+        hs = float(2.0 ** min(15, max(-15, int(np.floor(np.log2(headroom / hmax))))))
+        check(lib().cs_mlp_set_fp8_scale(self._h, C.c_float(hs)))
+        return hs
 
     def fit(self, genes, y, epochs: int, batch: int = 4096, lr: float = 1e-3, wd: float = 0.01, seed: int = 0,
             target_norm: bool = True):

@@ -49,12 +49,15 @@ __device__ __forceinline__ float gelu_grad_f(float x) {
 
 // Packed-f16 GELU of the MLP screen (two activations per lane and
 // instruction, no transcendentals).  gelu(x) = x S(x) with
-//   x16 = f16(x) (toward zero), v = x16 q, w = min(v v - 1, 1),
+//   x16 = f16(x) (toward zero), v = x16 q, w = v v - 1,
 //   S = clamp(1/2 + v R(w), 0, 1), R of degree 6 in Horner form;
-// q = f16(sqrt 2) / 4 maps |x| <= 4.0004 onto w in [-1, 1], beyond which the
-// FMA's clamp bit saturates S.  A power-of-two scale carried by x (FP8 screen)
-// is absorbed into q.  Coefficients and error bounds: gelu_poly_fit.py
-// (max |S - S_exact| 6.8e-4 with every step in f16).
+// q = f16(sqrt 2) / 4 maps |x| <= 4.0004 onto w in [-1, 1], the fitted range.
+// Beyond it R (even degree, positive leading coefficient) stays above
+// 1 / (2 |v|), up to f16 overflow to +inf, so the FMA's clamp bit saturates
+// S to 0 or 1 without clamping w first.  A power-of-two scale carried by x
+// (FP8 screen) is absorbed into q.  Coefficients and error bounds:
+// gelu_poly_fit.py (max |S - S_exact| 6.8e-4 with every step in f16, checked
+// out to the f16 range of x).
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
@@ -67,7 +70,7 @@ __device__ __forceinline__ h2 pk_rtz(float a, float b) { return __builtin_bit_ca
 template <typename V>
 __device__ __forceinline__ V gelu_sig_scaled(V v) {  // S from v = x q
     const V one = (V)(_Float16)1.0f;
-    const V w = __builtin_elementwise_min(__builtin_elementwise_fma(v, v, -one), one);
+    const V w = __builtin_elementwise_fma(v, v, -one);
     V p = (V)(_Float16)0.022354126f;
     p = __builtin_elementwise_fma(p, w, (V)(_Float16)-0.0514831543f);
     p = __builtin_elementwise_fma(p, w, (V)(_Float16)0.0627441406f);
@@ -89,9 +92,11 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 //   w1f  [16 m-tiles][64 lanes] x 4 bf16: A fragments of W1^T for
 //        v_mfma_f32_16x16x16_bf16 (lane l: hidden 16 m + l%16, genes 4(l/16)..+3)
 //   w2f  [8 k-blocks][16 n-tiles][64 lanes] x 8: B fragments of W2 for
-//        v_mfma_f32_16x16x32_{bf16,fp8} in the permuted K order that matches the
-//        layer-1 accumulator layout: slot e of lane group q in k-block kb is
-//        hidden unit 32 kb + (e < 4 ? 4 q + e : 16 + 4 q + e - 4)
+//        v_mfma_f32_16x16x32_bf16 (and, as 8 E4M3 bytes, of the FP8 screen's
+//        v_mfma_scale_f32_16x16x128_f8f6f4, four k-blocks per instruction) in
+//        the permuted K order that matches the layer-1 accumulator layout:
+//        slot e of lane group q in k-block kb is hidden unit
+//        32 kb + (e < 4 ? 4 q + e : 16 + 4 q + e - 4)
 __host__ __device__ constexpr int w2_k_of(int kb, int q, int e) {
     return 32 * kb + (e < 4 ? 4 * q + e : 16 + 4 * q + (e - 4));
 }
@@ -100,12 +105,12 @@ struct MlpScreenW {
     const bf16x4 *w1f;     // 16 * 64
     const bf16x4 *w1fh;    // 16 * 64, times h_scale (fp8 screen)
     const bf16x8 *w2f;     // 8 * 16 * 64 (bf16)
-    const uint64_t *w2f8;  // 8 * 16 * 64 (fp8 x 8)
+    const uint64_t *w2f8;  // 8 * 16 * 64 (fp8 x 8): GELU_Q W2 in per-block E8M0 scale units
+    const uint32_t *w2s8;  // [8 n-tile pairs][64 lanes]: the E8M0 bytes of those blocks
     const float *b1, *b2, *w3;
     float b3;
     float h_scale;   // fp8: power of two; W1, b1 (so H1) are multiplied by it
-    float w_scale;   // fp8: power of two; W2 was multiplied by it before conversion
-    float descale;   // q / (h_scale * w_scale), q = GELU_Q: FP8 layer-2 accumulators to v = q z2
+    int h_e8m0;      // fp8: E8M0 byte of 1 / h_scale, the layer-2 MFMA's H1 scale
     float q1;        // v = x q1 in the layer-1 GELU: GELU_Q / h_scale (fp8) or GELU_Q
     // FP64 scorer: W1 n-major [256][16], W2 as v_mfma_f64_16x16x4f64 A
     // fragments [n-step 64][m-tile 16][lane 64], b1 | b2 | w3, b3

@@ -202,7 +202,7 @@ __global__ void k_adamw(float *__restrict__ p, const float *__restrict__ g, floa
 // BF16 copies for training and the fragment layouts for the screen
 __global__ void k_refresh(const float *__restrict__ p, __bf16 *__restrict__ w1b, __bf16 *__restrict__ w2b,
                           bf16x4 *__restrict__ w1f, bf16x4 *__restrict__ w1fh, bf16x8 *__restrict__ w2f,
-                          uint64_t *__restrict__ w2f8, float h_scale, float w_scale) {
+                          uint64_t *__restrict__ w2f8, uint32_t *__restrict__ w2s8, float h_scale) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < MLP_IN * MLP_H) w1b[i] = (__bf16)p[P_W1 + i];
     if (i < MLP_H * MLP_H) w2b[i] = (__bf16)p[P_W2 + i];
@@ -221,15 +221,53 @@ __global__ void k_refresh(const float *__restrict__ p, __bf16 *__restrict__ w1b,
         const int kb = i / 1024, nt = (i / 64) % 16, lane = i % 64, q = lane >> 4;
         const int n = 16 * nt + (lane & 15);
         bf16x8 v;
-        uint64_t f8 = 0;
-        for (int e = 0; e < 8; ++e) {
-            const float w = p[P_W2 + w2_k_of(kb, q, e) * MLP_H + n];
-            v[e] = (__bf16)w;
-            const uint32_t pk = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(w * w_scale, 0.0f, 0, false);
-            f8 |= (uint64_t)(pk & 0xffu) << (8 * e);
-        }
+        for (int e = 0; e < 8; ++e) v[e] = (__bf16)p[P_W2 + w2_k_of(kb, q, e) * MLP_H + n];
         w2f[i] = v;
-        w2f8[i] = f8;
+    }
+    if (i < 2 * 16 * 64) {
+        // FP8 screen: GELU_Q W2 (the layer-2 GELU input scale, folded in as
+        // in the BF16 screen) in E4M3 with one E8M0 scale per block of 32 k
+        // values of an output unit.  In v_mfma_scale_f32_16x16x128_f8f6f4 the
+        // A bytes of lane group g sit at k = 16 g .. 16 g + 15 (its first four
+        // VGPRs) and 64 + 16 g .. (its last four), so scale block b (k = 32 b
+        // .. 32 b + 31, scale byte from lane 16 b + row) holds the 8-byte
+        // pieces j of the k-quad kq with j >> 1 = b >> 1 (pieces j = 0, 1 are
+        // the first four VGPRs) of lane groups q with q >> 1 = b & 1
+        // (measured on the device with one nonzero VGPR of one lane group at
+        // a time).  Thread (kq, nt, 16 b + r) scales block b of out unit
+        // 16 nt + r: the block holds its values divided by 2^e, e the smallest
+        // exponent that keeps the largest magnitude within E4M3's 448, and the
+        // lane's scale byte is e + 127.  w2s8[np][lane] packs the bytes of the
+        // four instructions of n-tile pair np: byte 2 kq + (nt & 1).
+        const int kq = i / 1024, nt = (i / 64) % 16, lane = i % 64, b = lane >> 4, r = lane & 15;
+        const int n = 16 * nt + r;
+        float w[32];  // [q - q0][j - j0][slot]
+        float mx = 0.0f;
+        const int q0 = 2 * (b & 1), j0 = 2 * (b >> 1);
+        for (int u = 0; u < 32; ++u) {
+            const int q = q0 + (u >> 4), j = j0 + ((u >> 3) & 1), e = u & 7;
+            w[u] = GELU_Q * p[P_W2 + w2_k_of(4 * kq + j, q, e) * MLP_H + n];
+            mx = fmaxf(mx, fabsf(w[u]));
+        }
+        int e = -127;
+        if (mx > 0.0f) {
+            int ex;
+            (void)frexpf(mx / 448.0f, &ex);  // mx / 448 = f 2^ex, f in [0.5, 1)
+            e = ex;
+            if (ldexpf(mx, -(e - 1)) <= 448.0f) e -= 1;
+            e = max(-127, min(127, e));
+        }
+        for (int pc = 0; pc < 4; ++pc) {  // the block's four 8-byte pieces
+            const int q = q0 + (pc >> 1), j = j0 + (pc & 1);
+            uint64_t f8 = 0;
+            for (int e8 = 0; e8 < 8; ++e8) {
+                const float x = ldexpf(w[8 * pc + e8], -e);
+                const uint32_t pk = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false);
+                f8 |= (uint64_t)(pk & 0xffu) << (8 * e8);
+            }
+            w2f8[((4 * kq + j) * 16 + nt) * 64 + 16 * q + r] = f8;
+        }
+        reinterpret_cast<uint8_t *>(w2s8)[((nt >> 1) * 64 + lane) * 4 + 2 * kq + (nt & 1)] = (uint8_t)(e + 127);
     }
 }
 
@@ -275,7 +313,7 @@ static int refresh(cs_mlp *m) {
     cs_ctx *ctx = m->ctx;
     hipLaunchKernelGGL(k_refresh, dim3(MLP_H * MLP_H / 256), dim3(256), 0, ctx->stream, (const float *)m->params.p,
                        (__bf16 *)m->w1b.p, (__bf16 *)m->w2b.p, (bf16x4 *)m->w1f.p, (bf16x4 *)m->w1fh.p,
-                       (bf16x8 *)m->w2f.p, (uint64_t *)m->w2f8.p, m->h_scale, m->w_scale);
+                       (bf16x8 *)m->w2f.p, (uint64_t *)m->w2f8.p, (uint32_t *)m->w2s8.p, m->h_scale);
     HIPCHK(hipGetLastError());
     return CS_OK;
 }
@@ -375,7 +413,7 @@ int mlp_setup(cs_mlp *m, uint64_t seed) {
         (rc = ensure(m->w1b, 2 * MLP_IN * MLP_H)) || (rc = ensure(m->w2b, 2 * MLP_H * MLP_H)) ||
         (rc = ensure(m->w1f, 8 * 16 * 64)) || (rc = ensure(m->w1fh, 8 * 16 * 64)) ||
         (rc = ensure(m->w2f, 16 * 8 * 16 * 64)) ||
-        (rc = ensure(m->w2f8, 8 * 8 * 16 * 64)))
+        (rc = ensure(m->w2f8, 8 * 8 * 16 * 64)) || (rc = ensure(m->w2s8, 4 * 8 * 64)))
         return rc;
     std::vector<float> p;
     init_params(seed, p);
@@ -406,7 +444,7 @@ int cs_mlp_create(cs_ctx *ctx, uint64_t seed, cs_mlp **out) {
 int cs_mlp_destroy(cs_mlp *m) {
     if (!m) return CS_OK;
     (void)hipStreamSynchronize(m->ctx->stream);
-    csf::DevBuf *bufs[] = {&m->params, &m->grads, &m->m, &m->v, &m->w1b, &m->w2b, &m->w1f, &m->w1fh, &m->w2f, &m->w2f8, &m->f64pack,
+    csf::DevBuf *bufs[] = {&m->params, &m->grads, &m->m, &m->v, &m->w1b, &m->w2b, &m->w1f, &m->w1fh, &m->w2f, &m->w2f8, &m->w2s8, &m->f64pack,
                            &m->xb, &m->z1, &m->h1b, &m->z2, &m->h2b, &m->out, &m->dout, &m->dz2f, &m->dz2b,
                            &m->dh1, &m->dz1f, &m->dz1b, &m->loss_part, &m->gws};
     for (auto *b : bufs)
@@ -432,13 +470,11 @@ int cs_mlp_set_params(cs_mlp *m, const float *host) {
     return CS_OK;
 }
 
-int cs_mlp_set_fp8_scales(cs_mlp *m, float h_scale, float w_scale) {
-    int eh = 0, ew = 0;
-    if (!m || !(h_scale > 0) || !(w_scale > 0) || std::frexp(h_scale, &eh) != 0.5f ||
-        std::frexp(w_scale, &ew) != 0.5f || eh < -14 || eh > 16 || ew < -14 || ew > 16)
-        return fail(CS_E_BADARGS, "fp8 scales must be powers of two in [2^-15, 2^15]");
+int cs_mlp_set_fp8_scale(cs_mlp *m, float h_scale) {
+    int eh = 0;
+    if (!m || !(h_scale > 0) || std::frexp(h_scale, &eh) != 0.5f || eh < -14 || eh > 16)
+        return fail(CS_E_BADARGS, "the fp8 H1 scale must be a power of two in [2^-15, 2^15]");
     m->h_scale = h_scale;
-    m->w_scale = w_scale;
     int rc;
     if ((rc = refresh(m))) return rc;
     HIPCHK(hipStreamSynchronize(m->ctx->stream));

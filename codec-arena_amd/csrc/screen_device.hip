@@ -473,6 +473,7 @@ struct ScoreArgs {
 // wave the candidates' genes (bf16) and their scores.
 constexpr size_t MLP_W2_BF16_BYTES = 8 * 16 * 64 * 16;
 constexpr size_t MLP_W2_FP8_BYTES = 8 * 16 * 64 * 8;
+constexpr size_t MLP_W2S_BYTES = 8 * 64 * 4;  // FP8: E8M0 scale bytes of the W2 fragments
 constexpr size_t MLP_VEC_BYTES = 2 * MLP_H * 4 + (MLP_H / 2) * 4;  // b1 | b2 | w3 as f16 pairs
 constexpr size_t MLP_WAVE_BYTES = 64 * MLP_IN * 2;  // genes (bf16), then the scores over them
 // waves per workgroup: the MLP variants run 8 (two per SIMD) around one LDS
@@ -489,8 +490,9 @@ constexpr size_t F64_WAVE_BYTES = 64 * (size_t)MLP_IN * 8 + 64 * 8;
 
 struct MlpLds {
     const void *w2;        // W2 fragments (bf16 or fp8)
+    const uint32_t *w2s;   // fp8: E8M0 block-scale bytes of the W2 fragments
     const bf16x4 *w1f;     // W1^T fragments, global (fp8 screen: times h_scale)
-    const float *b1, *b2;  // biases (fp8 screen: b1 h_scale, b2 h_scale w_scale)
+    const float *b1, *b2;  // biases (fp8 screen: b1 h_scale; both screens: b2 GELU_Q)
     const uint32_t *w3h;   // w3 as f16 pairs, [n-tile pair][lane group][4]
     __bf16 *xs;
     float *sc;
@@ -667,12 +669,16 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
             cb[c4] = bb;
         }
         if constexpr (SC == SC_MLP_FP8) {
-            // CDNA4's v_mfma_f32_16x16x128_f8f6f4 (E4M3 operands, no block
-            // scales): four k-blocks per instruction, W2 staged as 32-byte
-            // fragments [k-quad][n-tile][lane] and H1 as the four k-blocks'
-            // 8-byte fragments side by side, in the same k order.
-#pragma unroll
-            for (int m = 0; m < 2; ++m) {
+            // CDNA4's v_mfma_scale_f32_16x16x128_f8f6f4 (E4M3 operands): four
+            // k-blocks per instruction, W2 staged as 32-byte fragments
+            // [k-quad][n-tile][lane] and H1 as the four k-blocks' 8-byte
+            // fragments side by side, in the same k order.  The scale
+            // operands undo W2's per-block E8M0 scales (byte 2 m + (n-tile &
+            // 1) of this lane's word for pair np, laid out by k_refresh) and
+            // H1's h_scale, so the accumulators hold v = q z2.
+            const int sw = (int)ml.w2s[np * 64 + lane];
+            auto kquad = [&](auto mc) {  // mc: the k-quad, a compile-time constant (it selects scale bytes)
+                constexpr int m = decltype(mc)::value;
                 const i32x8 wa = reinterpret_cast<const i32x8 *>(ml.w2)[(m * 16 + 2 * np) * 64 + lane];
                 const i32x8 wb = reinterpret_cast<const i32x8 *>(ml.w2)[(m * 16 + 2 * np + 1) * 64 + lane];
 #pragma unroll
@@ -680,10 +686,14 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
                     const i32x8 hx = __builtin_bit_cast(
                         i32x8, u64x4{(uint64_t)af[c4][4 * m], (uint64_t)af[c4][4 * m + 1], (uint64_t)af[c4][4 * m + 2],
                                      (uint64_t)af[c4][4 * m + 3]});
-                    ca[c4] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(wa, hx, ca[c4], 0, 0, 0, 0, 0, 0);
-                    cb[c4] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(wb, hx, cb[c4], 0, 0, 0, 0, 0, 0);
+                    ca[c4] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(wa, hx, ca[c4], 0, 0, 2 * m, sw, 0,
+                                                                              w.h_e8m0);
+                    cb[c4] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(wb, hx, cb[c4], 0, 0, 2 * m + 1, sw,
+                                                                              0, w.h_e8m0);
                 }
-            }
+            };
+            kquad(std::integral_constant<int, 0>{});
+            kquad(std::integral_constant<int, 1>{});
         } else {
         // W2 fragments stream through per k-block: eight independent MFMAs each
 #pragma unroll
@@ -707,8 +717,8 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
             float z[8];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                z[r] = SC == SC_MLP_FP8 ? ca[c4][r] * w.descale : ca[c4][r];
-                z[4 + r] = SC == SC_MLP_FP8 ? cb[c4][r] * w.descale : cb[c4][r];
+                z[r] = ca[c4][r];
+                z[4 + r] = cb[c4][r];
             }
             h8 x;
 #pragma unroll
@@ -717,7 +727,7 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
                 x[2 * p2] = xp[0];
                 x[2 * p2 + 1] = xp[1];
             }
-            const h8 g = x * gelu_sig_scaled<h8>(x);  // x = q z2 (scale folded into W, b2 or the descale)
+            const h8 g = x * gelu_sig_scaled<h8>(x);  // x = q z2 (q folded into W2 and b2)
             float acc3 = part[c4];
             acc3 = __builtin_amdgcn_fdot2(h2{g[0], g[1]}, __builtin_bit_cast(h2, w3v.x), acc3, false);
             acc3 = __builtin_amdgcn_fdot2(h2{g[2], g[3]}, __builtin_bit_cast(h2, w3v.y), acc3, false);
@@ -1027,12 +1037,13 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
     }
     MlpLds ml{};
     if constexpr (SC == SC_MLP_BF16 || SC == SC_MLP_FP8) {
-        // after the rings: W2 fragments, b1 | b2 | w3 pairs, then per wave the
-        // genes (the scores reuse that buffer once layer 1 has read it).  The
-        // FP8 screen carries H1 scaled by the power of two h_scale (exact on
-        // W1 and b1) and Z2 by h_scale w_scale.
+        // after the rings: W2 fragments (FP8: then their scale bytes), b1 |
+        // b2 | w3 pairs, then per wave the genes (the scores reuse that
+        // buffer once layer 1 has read it).  The FP8 screen carries H1 scaled
+        // by the power of two h_scale (exact on W1 and b1); its layer-2 MFMAs
+        // undo that and W2's block scales.
         char *base = reinterpret_cast<char *>(smem + NW * 2 * QCAP);
-        const size_t w2b = SC == SC_MLP_FP8 ? MLP_W2_FP8_BYTES : MLP_W2_BF16_BYTES;
+        const size_t w2b = SC == SC_MLP_FP8 ? MLP_W2_FP8_BYTES + MLP_W2S_BYTES : MLP_W2_BF16_BYTES;
         const uint4 *src = SC == SC_MLP_FP8 ? reinterpret_cast<const uint4 *>(a.mlp.w2f8)
                                             : reinterpret_cast<const uint4 *>(a.mlp.w2f);
         if constexpr (SC == SC_MLP_FP8) {
@@ -1043,15 +1054,18 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
                 const uint32_t kb = i >> 10, nt = (i >> 6) & 15u, ln = i & 63u;
                 d8[(((kb >> 2) * 16u + nt) * 64u + ln) * 4u + (kb & 3u)] = s8[i];
             }
+            uint32_t *ds = reinterpret_cast<uint32_t *>(base + MLP_W2_FP8_BYTES);
+            for (uint32_t i = threadIdx.x; i < 8u * 64u; i += blockDim.x) ds[i] = a.mlp.w2s8[i];
         } else {
             uint4 *dst = reinterpret_cast<uint4 *>(base);
             for (uint32_t i = threadIdx.x; i < w2b / 16; i += blockDim.x) dst[i] = src[i];
         }
-        // The GELU input scale q: BF16 folds it into W1, b1, b2 and both
-        // screens 1/q into w3 (FP8 layer 2 applies q with the descale), so
-        // the accumulators hold v = x q directly
+        // The GELU input scale q: BF16 folds it into W1 and b1, so that H1
+        // carries it into layer 2; FP8 into W2 (its layer-1 GELU takes
+        // q1 = q / h_scale); both fold it into b2 and 1/q into w3, so the
+        // layer-2 accumulators hold v = z2 q directly
         const float hs = SC == SC_MLP_FP8 ? a.mlp.h_scale : GELU_Q;
-        const float zs = SC == SC_MLP_FP8 ? a.mlp.h_scale * a.mlp.w_scale : GELU_Q;
+        const float zs = GELU_Q;
         float *vec = reinterpret_cast<float *>(base + w2b);
         uint32_t *w3h = reinterpret_cast<uint32_t *>(vec + 2 * MLP_H);
         for (uint32_t i = threadIdx.x; i < MLP_H; i += blockDim.x) {
@@ -1068,6 +1082,7 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
         }
         char *wv = base + w2b + MLP_VEC_BYTES + (size_t)wave * MLP_WAVE_BYTES;
         ml.w2 = base;
+        ml.w2s = reinterpret_cast<const uint32_t *>(base + MLP_W2_FP8_BYTES);
         ml.w1f = SC == SC_MLP_FP8 ? a.mlp.w1fh : a.mlp.w1f;
         ml.b1 = vec;
         ml.b2 = vec + MLP_H;
@@ -1759,14 +1774,17 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         mw.w1fh = (const bf16x4 *)ml->w1fh.p;
         mw.w2f = (const bf16x8 *)ml->w2f.p;
         mw.w2f8 = (const uint64_t *)ml->w2f8.p;
+        mw.w2s8 = (const uint32_t *)ml->w2s8.p;
         mw.b1 = pp + P_B1;
         mw.b2 = pp + P_B2;
         mw.w3 = pp + P_W3;
         mw.b3 = b3;
         mw.h_scale = ml->h_scale;
-        mw.w_scale = ml->w_scale;
-        // FP8 layer 2: the descale also applies the GELU input scale q
-        mw.descale = GELU_Q / (ml->h_scale * ml->w_scale);
+        {
+            int eh = 0;
+            (void)std::frexp(ml->h_scale, &eh);  // h_scale = 2^(eh - 1)
+            mw.h_e8m0 = 127 - (eh - 1);
+        }
         mw.q1 = sc == SC_MLP_FP8 ? GELU_Q / ml->h_scale : GELU_Q;
         if (sc == SC_MLP_F64) {
             // FP64 copies of the FP32 master parameters, repacked per call
@@ -1815,7 +1833,7 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         h.smem = nw * 2 * QCAP * 4 + F64_W1_BYTES + F64_VEC_BYTES + nw * F64_WAVE_BYTES;
     } else {
         const size_t nw = score_waves<SC_MLP_BF16>();
-        h.smem = nw * 2 * QCAP * 4 + (sc == SC_MLP_FP8 ? MLP_W2_FP8_BYTES : MLP_W2_BF16_BYTES) + MLP_VEC_BYTES +
+        h.smem = nw * 2 * QCAP * 4 + (sc == SC_MLP_FP8 ? MLP_W2_FP8_BYTES + MLP_W2S_BYTES : MLP_W2_BF16_BYTES) + MLP_VEC_BYTES +
                  nw * MLP_WAVE_BYTES;
     }
     {

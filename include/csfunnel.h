@@ -172,7 +172,8 @@ int cs_set_offset_mode(cs_ctx *ctx, int mode);
  * A 16 -> 256 -> 256 -> 1 GELU MLP over the 16 legacy effector genes, trained
  * on the device with BF16 MFMA forward/backward kernels (FP32 accumulation,
  * FP32 master weights, AdamW as torch.optim.AdamW) and used as an alternative
- * screen scorer in BF16 or FP8 (E4M3, per-tensor scales). Gene and target
+ * screen scorer in BF16 or FP8 (E4M3: a per-tensor scale for the layer-2
+ * activations, per-block E8M0 scales for W2). Gene and target
  * pointers are DEVICE memory: genes [n][ldg] f32 (the first 16 columns are
  * the inputs), targets [n] f32. */
 typedef struct cs_mlp cs_mlp;
@@ -182,9 +183,11 @@ int cs_mlp_create(cs_ctx *ctx, uint64_t seed, cs_mlp **out);
 int cs_mlp_destroy(cs_mlp *mlp);
 int cs_mlp_get_params(cs_mlp *mlp, float *host_params);
 int cs_mlp_set_params(cs_mlp *mlp, const float *host_params);
-/* FP8 screen scales: H1 and W2 are multiplied by these before E4M3 rounding.
- * Both must be powers of two in [2^-15, 2^15] (else CS_E_BADARGS). */
-int cs_mlp_set_fp8_scales(cs_mlp *mlp, float h_scale, float w_scale);
+/* FP8 screen: H1 is multiplied by h_scale before E4M3 rounding; h_scale must
+ * be a power of two in [2^-15, 2^15] (else CS_E_BADARGS).  W2 needs no scale
+ * from the caller: it is rounded in blocks of 32 k values, each with its own
+ * E8M0 scale (the layer-2 MFMA's block scales). */
+int cs_mlp_set_fp8_scale(cs_mlp *mlp, float h_scale);
 /* out[n] (device) = MLP(genes); *h1max (optional) = max |H1| for FP8 calibration */
 int cs_mlp_forward(cs_mlp *mlp, const float *genes, int ldg, uint64_t n, float *out, float *h1max);
 /* loss = mean (MLP(genes) - y)^2 and its gradient (host_grads, optional) */

@@ -41,6 +41,12 @@ def test_f16_sequence_error_bounds():
     g_err = np.max(np.abs(g16 - x * fit.s_exact(x)))
     assert s_err < 1.2e-3, s_err
     assert g_err < 8e-3, g_err  # f16 resolution of x near |x| = 16
-    # the sigmoid factor saturates correctly far out
+    # the sigmoid factor saturates correctly far out, with w unclamped: over
+    # the whole f16 range of x (and beyond it, where the conversion toward
+    # zero stops at 65504) the error bound holds and nothing is NaN
     s_far, _ = fit.s_f16(np.array([-60.0, -8.0, 8.0, 60.0]), coef)
     assert np.allclose(s_far, [0.0, 0.0, 1.0, 1.0], atol=1e-3)
+    xf = np.concatenate([np.linspace(-70000.0, 70000.0, 280001), [-1e9, -65504.0, 65504.0, 1e9]])
+    sf, _ = fit.s_f16(xf, coef)
+    assert not np.isnan(sf).any()
+    assert np.max(np.abs(sf - fit.s_exact(xf))) < 1.2e-3
