@@ -4,8 +4,10 @@ Workload (production-shard quadratic screen): 128 elites, a 153-term
 quadratic surface, 2.65e9 candidates per generation in 4 splitmix64 streams,
 keep 128, jump 0.01, immigrants 0.01, sigma_g = "%.4f" % (0.12 * 0.93^g),
 seed_g = g; the survivors of each generation become the next generation's
-elites (ids 128 g + rank).  One step = one generation.  Inputs are generated
-on the device; only the 128 elites and the model cross PCIe per step.
+elites (ids 128 g + rank).  One step = one generation; the default 28 steps
+are the shard's whole run, generations 0 .. 27 (the warmup step runs
+generation 0 once more before the timed region).  Inputs are generated on
+the device; only the 128 elites and the model cross PCIe per step.
 
 Beside the headline the same line carries the top-k recall of each screen
 against FP64 scoring of the identical genomes (2^22 candidates of generation
@@ -193,35 +195,38 @@ def _rows(surv, lam):
 
 
 def run_generations(dev, model, elites, scorer, mlp, steps, warmup, keep):
-    """Chained generations (survivors -> next elites); returns seconds of the
-    timed steps and their stats."""
-    gen = 0
+    """Chained generations (survivors -> next elites) from `elites`: the
+    warmup runs generations 0 .. warmup - 1 untimed, then the timed steps run
+    generations 0 .. steps - 1 again from the same elites (the production
+    shard's run is 28 generations, seed g and sigma 0.12 0.93^g).  Returns
+    seconds of the timed steps and their stats."""
 
-    def step():
-        nonlocal elites, gen
-        cfg = _cfg(LAMBDA, keep, gen)
-        if scorer:
-            from csfunnel.mlp import screen_funnel
-            surv, screens = screen_funnel(dev, elites, cfg, scorer, mlp, rerank=4)
-        else:
-            surv, screens = dev.run(model, elites, cfg)
-        assert screens == LAMBDA and len(surv) == keep
-        st = dev.stats()
-        assert st["candidates"] == LAMBDA, st
-        elites = next_elites(surv, gen)
-        gen += 1
-        return st
+    def chain(n, timed):
+        el = elites
+        out = []
+        for gen in range(n):
+            cfg = _cfg(LAMBDA, keep, gen)
+            if scorer:
+                from csfunnel.mlp import screen_funnel
+                surv, screens = screen_funnel(dev, el, cfg, scorer, mlp, rerank=4)
+            else:
+                surv, screens = dev.run(model, el, cfg)
+            assert screens == LAMBDA and len(surv) == keep
+            st = dev.stats()
+            assert st["candidates"] == LAMBDA, st
+            el = next_elites(surv, gen)
+            out.append(st)
+            if timed:
+                print(f"gen {gen}: {st['ms_total']:.1f} ms (map {st['ms_map']:.1f}, scan {st['ms_scan']:.1f}, "
+                      f"score {st['ms_score']:.1f}, replay {st['ms_replay']:.1f}, records {st['records']})",
+                      file=sys.stderr, flush=True)
+            else:
+                print(f"warmup gen {gen}: {st['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+        return out
 
-    for _ in range(warmup):
-        st = step()
-        print(f"warmup gen {gen - 1}: {st['ms_total']:.1f} ms", file=sys.stderr, flush=True)
-    stats = []
+    chain(warmup, False)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        stats.append(step())
-        print(f"gen {gen - 1}: {stats[-1]['ms_total']:.1f} ms (map {stats[-1]['ms_map']:.1f}, "
-              f"scan {stats[-1]['ms_scan']:.1f}, score {stats[-1]['ms_score']:.1f}, "
-              f"replay {stats[-1]['ms_replay']:.1f}, records {stats[-1]['records']})", file=sys.stderr, flush=True)
+    stats = chain(steps, True)
     return time.perf_counter() - t0, stats
 
 
@@ -277,7 +282,7 @@ def cpu_model():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=28)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-sample", type=int, default=1 << 27,
                     help="candidates of the 4-thread CPU leg (the all-threads leg scales it by threads / 4)")
@@ -388,7 +393,8 @@ def main():
         "config": {"workload": "production-shard quadratic screen (2.65e9 candidates/generation, 128 elites, "
                                "153-term surface, keep 128, 4 streams, chained generations)",
                    "lambda": LAMBDA, "keep": KEEP, "threads": THREADS, "elites": 128, "jump": JUMP, "imm": IMM,
-                   "first_generation": args.warmup, "parallelism": "single GPU"},
+                   "generations": f"0..{args.steps - 1} (seed g, sigma 0.12 0.93^g; warmup repeats the first)",
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_score", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": SCORE_TRAFFIC_PER_LAUNCH,
                      "traffic_source": "profiles/screen_walk_offsets_2p65e9.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
@@ -419,6 +425,7 @@ def main():
     if not args.no_cpu:
         legs = {}
         nthr = max(1, min(os.cpu_count() or 1, 16))  # the GPU host's share: at most 16 threads
+        cpu_baseline(1 << 22, nthr)  # untimed: threads, pages and clocks warmed up once
         for t in sorted({THREADS, nthr}):
             sample = args.cpu_sample * max(1, t // THREADS)
             v, secs = cpu_baseline(sample, t)

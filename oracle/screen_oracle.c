@@ -460,7 +460,12 @@ static int heap_offer(heap_t *h, const or_slot *s) {
 
 /* -------------------------------------------------------------- worker --- */
 
-typedef struct {
+/* One job per stream (or chunk).  The jobs sit side by side in one array, so
+ * each is aligned to a cache line and the per-candidate count lives in a
+ * register: a line shared by two threads' jobs would otherwise bounce between
+ * their cores on every candidate, and how much that costs depends on where
+ * the scheduler puts the threads. */
+typedef struct __attribute__((aligned(64))) {
     const or_model *m;
     const or_genome *elites;
     int ne;
@@ -479,6 +484,13 @@ typedef struct {
     uint64_t n_emit, cap_emit;
 } job_t;
 
+/* zeroed jobs on cache-line boundaries (sizeof(job_t) is a multiple of 64) */
+static job_t *jobs_alloc(uint64_t n) {
+    job_t *j = aligned_alloc(64, (n ? n : 1) * sizeof(job_t));
+    if (j) memset(j, 0, (n ? n : 1) * sizeof(job_t));
+    return j;
+}
+
 /* src/screen.zig:225-255; the draw counter is tracked so that each kept slot
  * records where its candidate started. */
 static void run_job(job_t *j) {
@@ -486,7 +498,7 @@ static void run_job(job_t *j) {
     const int ne = j->ne;
     const float sigma = j->cfg->sigma, jump_p = j->cfg->jump_p, imm_p = j->cfg->imm_p;
     or_slot slot;
-    uint64_t start = j->start0;
+    uint64_t start = j->start0, screens = 0;
     for (uint64_t i = j->i0; i < j->i0 + j->my_lambda; i++) {
         const or_genome *ea = &j->elites[(sm_next(&s) >> 32) % (uint64_t)ne];
         const or_genome *eb = &j->elites[(sm_next(&s) >> 32) % (uint64_t)ne];
@@ -521,7 +533,7 @@ static void run_job(job_t *j) {
         slot.index = i;
         slot.start = start;
         start += used;
-        j->screens++;
+        screens++;
         if (heap_offer(&j->heap, &slot) && j->cap_emit) {
             if (j->n_emit == j->cap_emit) {
                 j->cap_emit *= 2;
@@ -530,6 +542,7 @@ static void run_job(job_t *j) {
             j->emit[j->n_emit++] = slot;
         }
     }
+    j->screens += screens;
 }
 
 typedef struct {
@@ -575,7 +588,7 @@ static int screen_jobs(const or_model *m, const or_genome *elites, int ne, const
     if (cfg->keep == 0) return OR_E_KEEP0;
     uint64_t T = cfg->threads < cfg->lambda ? cfg->threads : cfg->lambda;
     if (T < 1) T = 1;
-    job_t *jobs = calloc(T, sizeof(job_t));
+    job_t *jobs = jobs_alloc(T);
     uint64_t per = cfg->lambda / T, issued = 0;
     for (uint64_t t = 0; t < T; t++) {
         uint64_t my = (t == T - 1) ? cfg->lambda - issued : per;
@@ -672,7 +685,7 @@ int or_screen_chunked(const or_model *m, const or_genome *elites, int ne, const 
     if (T < 1) T = 1;
     const uint64_t per = cfg->lambda / T;
     const uint64_t nj = T * (uint64_t)chunks;
-    job_t *jobs = calloc(nj, sizeof(job_t));
+    job_t *jobs = jobs_alloc(nj);
     uint64_t issued = 0;
     for (uint64_t t = 0; t < T; t++) {
         const uint64_t my = (t == T - 1) ? cfg->lambda - issued : per;

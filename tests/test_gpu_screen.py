@@ -300,28 +300,35 @@ def _chain_elites(survivors, gen):
 
 
 @pytest.mark.timeout(1200)
-def test_production_chained_generation_bit_exact(oracle, golden):
-    # The production shard at full size: generation 0 (2.65e9 candidates,
-    # 128 elites, keep 128, 4 streams) on the GPU, its survivors become the
-    # elites of generation 1 (seed 1, sigma "%.4f" % (0.12 * 0.93)), and
-    # generation 1 on the GPU must equal the oracle's -- every survivor field
-    # and screens_count -- over all 2.65e9 candidates.
+@pytest.mark.parametrize("last", [1, 27])
+def test_production_chained_generation_bit_exact(oracle, golden, last):
+    # The production shard at full size and over its chain: generations
+    # 0 .. last - 1 (2.65e9 candidates each, 128 elites, keep 128, 4 streams,
+    # seed g, sigma "%.4f" % (0.12 * 0.93^g)) on the GPU, each generation's
+    # survivors the next one's elites; generation `last` on the GPU must then
+    # equal the oracle's -- every survivor field and screens_count -- over all
+    # 2.65e9 candidates.  last = 27 is the 28th and final generation of the
+    # shard's run, where sigma has annealed to 0.0171 and the elites are
+    # 27 generations of GPU survivors.
     m, e = _c0(golden, "c1_elites.txt")
     model, mo = scr.load_model(m), oracle.load_model(m)
     lam = 2_650_000_000
+    el, or_el = scr.load_elites(e), None
     with scr.Screen(0) as d:
-        cfg0 = scr.Cfg(lam, 128, 0, 4, float(np.float32(0.12)), float(np.float32(0.01)), float(np.float32(0.01)))
-        print("case chained production generation 0 (GPU)", flush=True)
-        s0, _ = d.run(model, scr.load_elites(e), cfg0)
-        el1, or_el1 = _chain_elites(s0, 0)
-        cfg1 = scr.Cfg(lam, 128, 1, 4, scr.parse_f32(f"{0.12 * 0.93:.4f}"), cfg0.jump_p, cfg0.imm_p)
-        print("case chained production generation 1 (GPU)", flush=True)
-        got, screens = d.run(model, el1, cfg1)
-        assert d.stats()["candidates"] == lam
-    print("case chained production generation 1 (oracle, 4 streams in 64 chunks each on 16 host threads)", flush=True)
-    ref, rscreens = oracle.screen_chunked(mo, or_el1, lam, 128, 1, 4, cfg1.sigma, cfg1.jump_p, cfg1.imm_p,
+        for g in range(last + 1):
+            sigma = scr.parse_f32(f"{0.12 * 0.93 ** g:.4f}")
+            cfg = scr.Cfg(lam, 128, g, 4, sigma, float(np.float32(0.01)), float(np.float32(0.01)))
+            print(f"case chained production generation {g} (GPU)", flush=True)
+            surv, screens = d.run(model, el, cfg)
+            assert d.stats()["candidates"] == lam and screens == lam
+            if g < last:
+                el, or_el = _chain_elites(surv, g)
+    got = surv
+    print(f"case chained production generation {last} (oracle, 4 streams in 64 chunks each on 16 host threads)",
+          flush=True)
+    ref, rscreens = oracle.screen_chunked(mo, or_el, lam, 128, last, 4, cfg.sigma, cfg.jump_p, cfg.imm_p,
                                           workers=16, chunks=64)
-    assert screens == rscreens == lam
+    assert rscreens == lam
     _same(got, ref)
 
 
