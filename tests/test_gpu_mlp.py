@@ -228,7 +228,12 @@ def test_mlp_screen_fp8_recall(dev, trained):
     print("case MLP screen fp8 lam=2^22 keep=64", flush=True)
     recall, dmax, _, _ = _screen_recall(dev, m, 2, 1 << 22, 64)
     print(f"FP8 MLP screen: top-64 recall vs FP64 = {recall:.4f}, max |score - fp64| = {dmax:.3e}", flush=True)
-    assert recall >= 0.5
+    # E4M3's 3-bit mantissa bounds the raw order (0.78 measured with W2 in
+    # per-block E8M0 scales and H1's scale in its conversion; per-tensor W2
+    # scales gave 0.56); a wrong block layout or scale puts scores off by
+    # powers of two and recall near 0
+    assert recall >= 0.7
+    assert dmax < 8.0
     frec = _funnel_recall(dev, m, 2, 1 << 22, 64)
     print(f"FP8 screen + FP64 re-rank of the top 256: top-64 recall vs FP64 = {frec:.4f}", flush=True)
     assert frec >= 0.95
