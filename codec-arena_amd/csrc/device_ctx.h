@@ -31,6 +31,9 @@ inline int ensure(DevBuf &b, size_t bytes) {
     hipError_t e = hipMalloc(&b.p, want);
     if (e != hipSuccess) {
         b.p = nullptr;
+        // the runtime keeps the error as its last one: clear it, or the next
+        // launch check of this context would report it again
+        (void)hipGetLastError();
         return fail(CS_E_OOM, "hipMalloc(" + std::to_string(want) + "): " + hipGetErrorString(e));
     }
     b.bytes = want;

@@ -170,6 +170,20 @@ def test_keep_zero_and_empty(dev, golden):
     assert len(got) == 0 and screens == 0
 
 
+def test_allocation_failure_is_out_of_memory_and_the_context_recovers(dev, oracle, golden):
+    # 2^60 candidates need about 9 PB of block entries: the device allocation
+    # fails before any launch and the screen reports OutOfMemory; the same
+    # context then screens normally (the failed allocation leaves no error
+    # state behind for the next call's launch checks).
+    m, e = _c0(golden)
+    cfg = scr.Cfg(1 << 60, 16, 0, 4, float(np.float32(0.12)), float(np.float32(0.01)), float(np.float32(0.01)))
+    print(f"case mode={dev.mode} lam=2^60 (allocation must fail)", flush=True)
+    with pytest.raises(CsError) as ei:
+        dev.run(scr.load_model(m), scr.load_elites(e), cfg)
+    assert ei.value.name == "OutOfMemory", str(ei.value)
+    _run_both(dev, oracle, m, e, 1 << 16, 64, 3, 4, 0.12, 0.01, 0.01)
+
+
 def test_large_lambda_parity(dev, oracle, golden):
     # 2^26 candidates in 4 streams: every survivor bit-exact against the oracle
     m, e = _c0(golden, "c1_elites.txt")
