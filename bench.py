@@ -46,7 +46,7 @@ PUBLISHED_CPU = 17.2e6    # reference Zig SIMD screen, 4 threads (docs/SCALE.md:
 # (the profile names the source it was measured on; the line reports whether
 # that source is still the one built)
 PROFILED_SOURCE = "codec-arena_amd/csrc/screen_device.hip"
-PROFILED_SHA16 = "9afb84f2ba56947c"
+PROFILED_SHA16 = "9263b5000c910e19"
 VALU_LANE_INSTR = 1859
 VALU_PEAK = 256 * 128 * 2.4e9  # lane-instructions/s: 256 CUs x 128 lanes/cycle x 2.4 GHz
 SCORE_TRAFFIC_PER_LAUNCH = 9.567e6

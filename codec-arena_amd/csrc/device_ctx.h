@@ -71,7 +71,7 @@ struct cs_ctx {
 // activations of the last training batch.
 struct cs_mlp {
     cs_ctx *ctx = nullptr;
-    csf::DevBuf params, grads, m, v, w1b, w2b, w1f, w2f, w2f8, w2s8, f64pack;
+    csf::DevBuf params, grads, m, v, w1b, w2b, w1f, w1fh, w2f, w2f8, w2s8, f64pack;
     csf::DevBuf xb, z1, h1b, z2, h2b, out, dout, dz2f, dz2b, dh1, dz1f, dz1b, loss_part, gws;
     int cap = 0;
     long step = 0;

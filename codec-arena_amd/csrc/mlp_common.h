@@ -54,7 +54,8 @@ __device__ __forceinline__ float gelu_grad_f(float x) {
 // q = f16(sqrt 2) / 4 maps |x| <= 4.0004 onto w in [-1, 1], the fitted range.
 // Beyond it R (even degree, positive leading coefficient) stays above
 // 1 / (2 |v|), up to f16 overflow to +inf, so the FMA's clamp bit saturates
-// S to 0 or 1 without clamping w first.  Coefficients and error bounds:
+// S to 0 or 1 without clamping w first.  A power-of-two scale carried by x
+// (FP8 screen) is absorbed into q.  Coefficients and error bounds:
 // gelu_poly_fit.py (max |S - S_exact| 6.8e-4 with every step in f16, checked
 // out to the f16 range of x).
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
@@ -80,6 +81,11 @@ __device__ __forceinline__ V gelu_sig_scaled(V v) {  // S from v = x q
     const V sg = __builtin_elementwise_fma(v, p, (V)(_Float16)0.5f);
     return __builtin_elementwise_min(__builtin_elementwise_max(sg, (V)(_Float16)0.0f), one);
 }
+template <typename V>
+__device__ __forceinline__ V gelu_sig_v(V x, V q) {
+    return gelu_sig_scaled<V>(x * q);
+}
+__device__ __forceinline__ h2 gelu_sig_h2(h2 x, h2 q) { return gelu_sig_v<h2>(x, q); }
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
 // Layouts prepared for the screen (one wavefront scores 64 candidates):
@@ -97,14 +103,15 @@ __host__ __device__ constexpr int w2_k_of(int kb, int q, int e) {
 
 struct MlpScreenW {
     const bf16x4 *w1f;     // 16 * 64
+    const bf16x4 *w1fh;    // 16 * 64, times h_scale (fp8 screen)
     const bf16x8 *w2f;     // 8 * 16 * 64 (bf16)
     const uint64_t *w2f8;  // 8 * 16 * 64 (fp8 x 8): GELU_Q W2 in per-block E8M0 scale units
     const uint32_t *w2s8;  // [8 n-tile pairs][64 lanes]: the E8M0 bytes of those blocks
     const float *b1, *b2, *w3;
     float b3;
-    float h_scale;   // fp8: power of two; H1 is multiplied by it as it is rounded to E4M3
-    float h_div;     // fp8: 1 / h_scale, the divisor v_cvt_scalef32_pk_fp8_f16 applies
+    float h_scale;   // fp8: power of two; W1, b1 (so H1) are multiplied by it
     int h_e8m0;      // fp8: E8M0 byte of 1 / h_scale, the layer-2 MFMA's H1 scale
+    float q1;        // v = x q1 in the layer-1 GELU: GELU_Q / h_scale (fp8) or GELU_Q
     // FP64 scorer: W1 n-major [256][16], W2 as v_mfma_f64_16x16x4f64 A
     // fragments [n-step 64][m-tile 16][lane 64], b1 | b2 | w3, b3
     const double *w1d, *w2d, *vd;

@@ -201,17 +201,21 @@ __global__ void k_adamw(float *__restrict__ p, const float *__restrict__ g, floa
 
 // BF16 copies for training and the fragment layouts for the screen
 __global__ void k_refresh(const float *__restrict__ p, __bf16 *__restrict__ w1b, __bf16 *__restrict__ w2b,
-                          bf16x4 *__restrict__ w1f, bf16x8 *__restrict__ w2f, uint64_t *__restrict__ w2f8,
-                          uint32_t *__restrict__ w2s8) {
+                          bf16x4 *__restrict__ w1f, bf16x4 *__restrict__ w1fh, bf16x8 *__restrict__ w2f,
+                          uint64_t *__restrict__ w2f8, uint32_t *__restrict__ w2s8, float h_scale) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < MLP_IN * MLP_H) w1b[i] = (__bf16)p[P_W1 + i];
     if (i < MLP_H * MLP_H) w2b[i] = (__bf16)p[P_W2 + i];
     if (i < 16 * 64) {  // w1f[mt][lane]: hidden 16 mt + lane%16, genes 4 (lane/16) + e
         const int mt = i / 64, lane = i % 64, h = 16 * mt + (lane & 15), k0 = 4 * (lane >> 4);
-        bf16x4 v;
-        for (int e = 0; e < 4; ++e)  // the layer-1 GELU input scale, folded in (both screens)
-            v[e] = (__bf16)(p[P_W1 + (k0 + e) * MLP_H + h] * GELU_Q);
+        bf16x4 v, vh;
+        for (int e = 0; e < 4; ++e) {
+            const float w1 = p[P_W1 + (k0 + e) * MLP_H + h];
+            v[e] = (__bf16)(w1 * GELU_Q);  // the BF16 screen's layer-1 GELU input scale, folded in
+            vh[e] = (__bf16)((float)(__bf16)w1 * h_scale);  // exact: h_scale is a power of two
+        }
         w1f[i] = v;
+        w1fh[i] = vh;
     }
     if (i < 8 * 16 * 64) {  // w2f[kb][nt][lane]: out unit 16 nt + lane%16, permuted k slots
         const int kb = i / 1024, nt = (i / 64) % 16, lane = i % 64, q = lane >> 4;
@@ -221,8 +225,9 @@ __global__ void k_refresh(const float *__restrict__ p, __bf16 *__restrict__ w1b,
         w2f[i] = v;
     }
     if (i < 2 * 16 * 64) {
-        // FP8 screen: W2 in E4M3 with one E8M0 scale per block of 32 k
-        // values of an output unit (H1 carries the layer-2 GELU input scale).  In v_mfma_scale_f32_16x16x128_f8f6f4 the
+        // FP8 screen: GELU_Q W2 (the layer-2 GELU input scale, folded in as
+        // in the BF16 screen) in E4M3 with one E8M0 scale per block of 32 k
+        // values of an output unit.  In v_mfma_scale_f32_16x16x128_f8f6f4 the
         // A bytes of lane group g sit at k = 16 g .. 16 g + 15 (its first four
         // VGPRs) and 64 + 16 g .. (its last four), so scale block b (k = 32 b
         // .. 32 b + 31, scale byte from lane 16 b + row) holds the 8-byte
@@ -241,7 +246,7 @@ __global__ void k_refresh(const float *__restrict__ p, __bf16 *__restrict__ w1b,
         const int q0 = 2 * (b & 1), j0 = 2 * (b >> 1);
         for (int u = 0; u < 32; ++u) {
             const int q = q0 + (u >> 4), j = j0 + ((u >> 3) & 1), e = u & 7;
-            w[u] = p[P_W2 + w2_k_of(4 * kq + j, q, e) * MLP_H + n];
+            w[u] = GELU_Q * p[P_W2 + w2_k_of(4 * kq + j, q, e) * MLP_H + n];
             mx = fmaxf(mx, fabsf(w[u]));
         }
         int e = -127;
@@ -307,8 +312,8 @@ static void launch_gemm_wgrad(hipStream_t s, int M, int N, int K, const __bf16 *
 static int refresh(cs_mlp *m) {
     cs_ctx *ctx = m->ctx;
     hipLaunchKernelGGL(k_refresh, dim3(MLP_H * MLP_H / 256), dim3(256), 0, ctx->stream, (const float *)m->params.p,
-                       (__bf16 *)m->w1b.p, (__bf16 *)m->w2b.p, (bf16x4 *)m->w1f.p, (bf16x8 *)m->w2f.p,
-                       (uint64_t *)m->w2f8.p, (uint32_t *)m->w2s8.p);
+                       (__bf16 *)m->w1b.p, (__bf16 *)m->w2b.p, (bf16x4 *)m->w1f.p, (bf16x4 *)m->w1fh.p,
+                       (bf16x8 *)m->w2f.p, (uint64_t *)m->w2f8.p, (uint32_t *)m->w2s8.p, m->h_scale);
     HIPCHK(hipGetLastError());
     return CS_OK;
 }
@@ -406,7 +411,7 @@ int mlp_setup(cs_mlp *m, uint64_t seed) {
     if ((rc = ensure(m->params, 4 * P_TOTAL)) || (rc = ensure(m->grads, 4 * P_TOTAL)) ||
         (rc = ensure(m->m, 4 * P_TOTAL)) || (rc = ensure(m->v, 4 * P_TOTAL)) ||
         (rc = ensure(m->w1b, 2 * MLP_IN * MLP_H)) || (rc = ensure(m->w2b, 2 * MLP_H * MLP_H)) ||
-        (rc = ensure(m->w1f, 8 * 16 * 64)) ||
+        (rc = ensure(m->w1f, 8 * 16 * 64)) || (rc = ensure(m->w1fh, 8 * 16 * 64)) ||
         (rc = ensure(m->w2f, 16 * 8 * 16 * 64)) ||
         (rc = ensure(m->w2f8, 8 * 8 * 16 * 64)) || (rc = ensure(m->w2s8, 4 * 8 * 64)))
         return rc;
@@ -439,7 +444,7 @@ int cs_mlp_create(cs_ctx *ctx, uint64_t seed, cs_mlp **out) {
 int cs_mlp_destroy(cs_mlp *m) {
     if (!m) return CS_OK;
     (void)hipStreamSynchronize(m->ctx->stream);
-    csf::DevBuf *bufs[] = {&m->params, &m->grads, &m->m, &m->v, &m->w1b, &m->w2b, &m->w1f, &m->w2f, &m->w2f8, &m->w2s8, &m->f64pack,
+    csf::DevBuf *bufs[] = {&m->params, &m->grads, &m->m, &m->v, &m->w1b, &m->w2b, &m->w1f, &m->w1fh, &m->w2f, &m->w2f8, &m->w2s8, &m->f64pack,
                            &m->xb, &m->z1, &m->h1b, &m->z2, &m->h2b, &m->out, &m->dout, &m->dz2f, &m->dz2b,
                            &m->dh1, &m->dz1f, &m->dz1b, &m->loss_part, &m->gws};
     for (auto *b : bufs)
@@ -469,7 +474,10 @@ int cs_mlp_set_fp8_scale(cs_mlp *m, float h_scale) {
     int eh = 0;
     if (!m || !(h_scale > 0) || std::frexp(h_scale, &eh) != 0.5f || eh < -14 || eh > 16)
         return fail(CS_E_BADARGS, "the fp8 H1 scale must be a power of two in [2^-15, 2^15]");
-    m->h_scale = h_scale;  // applied by the screen as H1 is rounded to E4M3: no fragment changes
+    m->h_scale = h_scale;
+    int rc;
+    if ((rc = refresh(m))) return rc;
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
     return CS_OK;
 }
 

@@ -491,8 +491,8 @@ constexpr size_t F64_WAVE_BYTES = 64 * (size_t)MLP_IN * 8 + 64 * 8;
 struct MlpLds {
     const void *w2;        // W2 fragments (bf16 or fp8)
     const uint32_t *w2s;   // fp8: E8M0 block-scale bytes of the W2 fragments
-    const bf16x4 *w1f;     // W1^T fragments, global, times GELU_Q
-    const float *b1, *b2;  // biases, times GELU_Q
+    const bf16x4 *w1f;     // W1^T fragments, global (fp8 screen: times h_scale)
+    const float *b1, *b2;  // biases (fp8 screen: b1 h_scale; both screens: b2 GELU_Q)
     const uint32_t *w3h;   // w3 as f16 pairs, [n-tile pair][lane group][4]
     __bf16 *xs;
     float *sc;
@@ -604,6 +604,7 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
     bf16x4 w1v[16];
 #pragma unroll
     for (int mt = 0; mt < 16; ++mt) w1v[mt] = w1f[mt * 64 + lane];
+    const h8 q1v = (h8)(_Float16)w.q1;
     Frag af[4][8];
 #pragma unroll
     for (int c4 = 0; c4 < 4; ++c4) {
@@ -628,18 +629,22 @@ __device__ __forceinline__ float mlp_wave(const MlpScreenW &w, const MlpLds &ml,
                 xb[2 * t] = x23[0];
                 xb[2 * t + 1] = x23[1];
             }
-            // xa, xb are v = x q; H1 = v S(v) carries the factor q on to layer 2
-            const h8 ga = xa * gelu_sig_scaled<h8>(xa);
-            const h8 gb = xb * gelu_sig_scaled<h8>(xb);
+            h8 ga, gb;
+            if constexpr (SC == SC_MLP_FP8) {
+                ga = xa * gelu_sig_v<h8>(xa, q1v);
+                gb = xb * gelu_sig_v<h8>(xb, q1v);
+            } else {  // xa, xb are v = x q; H1 carries the factor q on to layer 2
+                ga = xa * gelu_sig_scaled<h8>(xa);
+                gb = xb * gelu_sig_scaled<h8>(xb);
+            }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
             const int mt = m4 + t;
             const h2 g01 = h2{ga[2 * t], ga[2 * t + 1]}, g23 = h2{gb[2 * t], gb[2 * t + 1]};
             if constexpr (SC == SC_MLP_FP8) {
                 typedef short s16x2 __attribute__((ext_vector_type(2)));
-                // the conversion divides by its scale operand: E4M3 of h_scale H1
-                s16x2 pk = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(s16x2{0, 0}, g01, w.h_div, false);
-                pk = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(pk, g23, w.h_div, true);
+                s16x2 pk = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(s16x2{0, 0}, g01, 1.0f, false);
+                pk = __builtin_amdgcn_cvt_scalef32_pk_fp8_f16(pk, g23, 1.0f, true);
                 const uint64_t u = __builtin_bit_cast(uint32_t, pk);
                 if (mt & 1) af[c4][mt >> 1] |= u << 32;
                 else af[c4][mt >> 1] = u;
@@ -1046,9 +1051,9 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
     if constexpr (SC == SC_MLP_BF16 || SC == SC_MLP_FP8) {
         // after the rings: W2 fragments (FP8: then their scale bytes), b1 |
         // b2 | w3 pairs, then per wave the genes (the scores reuse that
-        // buffer once layer 1 has read it).  The FP8 screen rounds H1 to
-        // E4M3 times the power of two h_scale; its layer-2 MFMAs undo that
-        // and W2's block scales.
+        // buffer once layer 1 has read it).  The FP8 screen carries H1 scaled
+        // by the power of two h_scale (exact on W1 and b1); its layer-2 MFMAs
+        // undo that and W2's block scales.
         char *base = reinterpret_cast<char *>(smem + NW * 2 * QCAP);
         const size_t w2b = SC == SC_MLP_FP8 ? MLP_W2_FP8_BYTES + MLP_W2S_BYTES : MLP_W2_BF16_BYTES;
         const uint4 *src = SC == SC_MLP_FP8 ? reinterpret_cast<const uint4 *>(a.mlp.w2f8)
@@ -1067,10 +1072,11 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             uint4 *dst = reinterpret_cast<uint4 *>(base);
             for (uint32_t i = threadIdx.x; i < w2b / 16; i += blockDim.x) dst[i] = src[i];
         }
-        // The GELU input scale q is folded into W1 and b1, so that H1
-        // carries it into layer 2, and into b2, and 1/q into w3: the layer-2
-        // accumulators hold v = z2 q directly
-        const float hs = GELU_Q;
+        // The GELU input scale q: BF16 folds it into W1 and b1, so that H1
+        // carries it into layer 2; FP8 into W2 (its layer-1 GELU takes
+        // q1 = q / h_scale); both fold it into b2 and 1/q into w3, so the
+        // layer-2 accumulators hold v = z2 q directly
+        const float hs = SC == SC_MLP_FP8 ? a.mlp.h_scale : GELU_Q;
         const float zs = GELU_Q;
         float *vec = reinterpret_cast<float *>(base + w2b);
         uint32_t *w3h = reinterpret_cast<uint32_t *>(vec + 2 * MLP_H);
@@ -1089,7 +1095,7 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
         char *wv = base + w2b + MLP_VEC_BYTES + (size_t)wave * MLP_WAVE_BYTES;
         ml.w2 = base;
         ml.w2s = reinterpret_cast<const uint32_t *>(base + MLP_W2_FP8_BYTES);
-        ml.w1f = a.mlp.w1f;
+        ml.w1f = SC == SC_MLP_FP8 ? a.mlp.w1fh : a.mlp.w1f;
         ml.b1 = vec;
         ml.b2 = vec + MLP_H;
         ml.w3h = w3h;
@@ -1777,6 +1783,7 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
         HIPCHK(hipStreamSynchronize(ctx->stream));
         MlpScreenW &mw = h.sa.mlp;
         mw.w1f = (const bf16x4 *)ml->w1f.p;
+        mw.w1fh = (const bf16x4 *)ml->w1fh.p;
         mw.w2f = (const bf16x8 *)ml->w2f.p;
         mw.w2f8 = (const uint64_t *)ml->w2f8.p;
         mw.w2s8 = (const uint32_t *)ml->w2s8.p;
@@ -1790,7 +1797,7 @@ static int screen_impl(cs_ctx *ctx, const cs_model *model, const cs_genome *elit
             (void)std::frexp(ml->h_scale, &eh);  // h_scale = 2^(eh - 1)
             mw.h_e8m0 = 127 - (eh - 1);
         }
-        mw.h_div = 1.0f / ml->h_scale;  // exact: a power of two
+        mw.q1 = sc == SC_MLP_FP8 ? GELU_Q / ml->h_scale : GELU_Q;
         if (sc == SC_MLP_F64) {
             // FP64 copies of the FP32 master parameters, repacked per call
             const size_t nd = (size_t)MLP_H * MLP_H + (size_t)MLP_H * MLP_IN + 3 * MLP_H;
