@@ -225,9 +225,9 @@ __global__ void k_refresh(const float *__restrict__ p, __bf16 *__restrict__ w1b,
         w2f[i] = v;
     }
     if (i < 2 * 16 * 64) {
-        // FP8 screen: GELU_Q W2 (the layer-2 GELU input scale, folded in as
-        // in the BF16 screen) in E4M3 with one E8M0 scale per block of 32 k
-        // values of an output unit.  In v_mfma_scale_f32_16x16x128_f8f6f4 the
+        // FP8 screen: GELU_Q W2 (the layer-2 GELU input scale, which the BF16
+        // screen's H1 carries instead) in E4M3 with one E8M0 scale per block
+        // of 32 k values of an output unit.  In v_mfma_scale_f32_16x16x128_f8f6f4 the
         // A bytes of lane group g sit at k = 16 g .. 16 g + 15 (its first four
         // VGPRs) and 64 + 16 g .. (its last four), so scale block b (k = 32 b
         // .. 32 b + 31, scale byte from lane 16 b + row) holds the 8-byte
