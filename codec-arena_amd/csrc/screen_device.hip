@@ -104,15 +104,18 @@ __device__ inline int lds_model_src(int i) {
 // ------------------------------------------------------------ splitmix64
 
 // state after n increments is seed + n*gamma; draw n is mix(that).  The
-// mixer is written on 32-bit halves so that each 64-bit shift is one
-// alignbit + one shift and each 64x64 product keeps only the words needed.
+// mixer is written on 32-bit halves so that each 64x64 product keeps only the
+// words needed.  The state arrives as a register pair (one v_lshl_add_u64
+// from the generation's base), so its shift is one v_lshrrev_b64 (one
+// instruction where alignbit + shift are two); the product's halves come
+// from separate instructions, so its shift stays alignbit + shift (a 64-bit
+// shift would first need moves to pair them).
 struct Mid {
     uint32_t lo, hi;  // u = x ^ (x >> 27), x = (s ^ (s >> 30)) * MIX1
 };
 __device__ __forceinline__ Mid mix_mid(uint64_t s) {
-    const uint32_t lo = (uint32_t)s, hi = (uint32_t)(s >> 32);
-    const uint32_t tlo = lo ^ __builtin_amdgcn_alignbit(hi, lo, 30);
-    const uint32_t thi = hi ^ (hi >> 30);
+    const uint64_t t = s ^ (s >> 30);
+    const uint32_t tlo = (uint32_t)t, thi = (uint32_t)(t >> 32);
     const uint64_t p = (uint64_t)tlo * (uint32_t)MIX1;
     const uint32_t xlo = (uint32_t)p;
     const uint32_t xhi = (uint32_t)(p >> 32) + tlo * (uint32_t)(MIX1 >> 32) + thi * (uint32_t)MIX1;
@@ -152,6 +155,11 @@ __device__ __forceinline__ float unit_f(uint32_t m) { return (float)m * 0x1p-24f
 // computation" (2019): M = floor((2^64-1)/d) + 1.
 __device__ __forceinline__ uint32_t fastmod(uint32_t a, uint64_t M, uint32_t d) {
     return (uint32_t)__umul64hi(M * (uint64_t)a, (uint64_t)d);
+}
+
+// base + the number of bits of m below this lane (two v_mbcnt)
+__device__ __forceinline__ uint32_t below(uint64_t m, uint32_t base) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base));
 }
 
 __device__ __forceinline__ int find_seg(const uint64_t *first, int T, uint64_t f) {
@@ -342,6 +350,7 @@ __global__ __launch_bounds__(64) void k_down(const uint64_t *__restrict__ in,
 // CSFUNNEL_WALK_SEG_LOG in [18, 30] at context creation fixes 2^seg_log
 // draws per segment instead.
 constexpr uint64_t MIN_SEG_BLOCKS = 256;
+constexpr uint64_t MAX_SEG_BLOCKS = 1ull << (30 - BLOCK_LOG);  // 2^30 draws
 constexpr uint64_t WALK_WAVES_PER_SIMD = 8;
 constexpr uint64_t BURN = 1ull << 22;  // ~14x the mean merge distance
 
@@ -359,46 +368,56 @@ struct SegOut {
 // 64-bit add per lane.  A chain that passes seg_end without resolving it
 // (impossible: a round spans < 4032 draws) stores y = ~0 and stops, which the
 // host's chain check rejects.
+//
+// The tests on positions use p = c - seg_start and b = B - seg_start as
+// signed 32-bit values (the scalar unit compares 32-bit values only, so
+// 64-bit ones go through the VALU): a segment spans at most 2^30 draws (the
+// host caps seg_blocks), the walk starts at most BURN draws before it and
+// stops within 4096 + 4032 draws after its end.
 __device__ void walk_segment(uint64_t seed, uint64_t c, uint64_t seg_start, uint64_t seg_end, uint32_t immT,
                              uint32_t jumpT, uint64_t *__restrict__ entries_seg, SegOut *__restrict__ out,
                              int lane) {
     uint64_t lane_step = (uint64_t)(62u * (uint32_t)lane) * GAMMA;
     asm volatile("" : "+v"(lane_step));  // opaque: keeps (c + 3) gamma + lane_step from being refactored
-    uint64_t B = seg_start;  // next block boundary to resolve
-    uint64_t r = 0;          // starts >= seg_start seen so far
+    const int32_t e = (int32_t)(seg_end - seg_start);
+    int32_t p = (int32_t)(int64_t)(c - seg_start);
+    int32_t b = 0;    // next block boundary to resolve, B - seg_start
+    uint64_t r = 0;   // starts >= seg_start seen so far
     for (;;) {
         const uint64_t x3 = seed + (c + 3) * GAMMA + lane_step;  // draw c + 62 j + 3
         const bool im = draw_u24(x3) < immT;
         // the jump test: draw c + 62 j + 33 (immigrant) or + 62
         const bool jp = draw_u24(x3 + (im ? 30ull * GAMMA : 59ull * GAMMA)) < jumpT;
         const uint32_t L = im ? (jp ? 35u : 33u) : (jp ? 64u : 62u);
-        const uint64_t stop = __ballot(im || jp);
+        const uint64_t stop = __ballot(im) | __ballot(jp);
         const uint32_t n = stop ? (uint32_t)__ffsll((unsigned long long)stop) : 64u;
         // valid starts this round: c + 62 k, k < n (ascending); the last one
-        const uint64_t last = c + 62ull * (n - 1);
+        const int32_t last = p + 62 * (int32_t)(n - 1);
         // lanes before the segment start do not count
         uint32_t ks = 0;
-        if (c < seg_start) ks = last < seg_start ? n : (uint32_t)(seg_start - c + 61) / 62u;
+        if (p < 0) ks = last < 0 ? n : (uint32_t)(61 - p) / 62u;
         // a round spans < 4032 draws, so at most one boundary falls inside
         // it; B <= last <=> the first start >= B is one of this round's
-        if (B <= last) {
-            const uint32_t kb = c >= B ? 0u : (uint32_t)(B - c + 61) / 62u;
-            const uint64_t first = c + 62ull * kb;
+        if (b <= last) {
+            const uint32_t kb = p >= b ? 0u : (uint32_t)(b - p + 61) / 62u;
+            const int32_t first = p + 62 * (int32_t)kb;  // >= b >= 0
             const uint64_t rel = r + (kb - min(kb, ks));
-            if (B == seg_end) {
+            if (b == e) {
                 if (lane == 0) {
-                    out->y = first;
+                    out->y = seg_start + (uint32_t)first;
                     out->total = rel;
                 }
                 return;
             }
-            if (B == seg_start && lane == 0) out->x = first;
-            if (lane == 0) entries_seg[(B - seg_start) >> BLOCK_LOG] = (rel << 8) | (first - B);
-            B += BLOCK;
+            if (b == 0 && lane == 0) out->x = seg_start + (uint32_t)first;
+            if (lane == 0) entries_seg[(uint32_t)b >> BLOCK_LOG] = (rel << 8) | (uint32_t)(first - b);
+            b += (int32_t)BLOCK;
         }
         r += n - ks;
-        c = last + (uint64_t)__builtin_amdgcn_readlane(L, (int)(n - 1));
-        if (c > seg_end + 4096) {
+        const uint32_t step = 62u * (n - 1) + (uint32_t)__builtin_amdgcn_readlane(L, (int)(n - 1));
+        c += step;
+        p += (int32_t)step;
+        if (p > e + 4096) {
             if (lane == 0) {
                 out->y = ~0ull;
                 out->total = 0;
@@ -941,7 +960,15 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
     // fetched per batch through an opaque uniform pointer so they stream
     // through scalar loads instead of being pinned (and spilled) for the
     // whole kernel.
-    float S = 0.0f, r = 0.0f;
+    //
+    // The reference starts each lane's sum and the tail at +0; here they
+    // start at their first product, which saves 17 adds.  Every partial
+    // result is then the reference's or both are zeros (adding a zero to a
+    // nonzero value returns that value exactly, so the two can differ only
+    // in the sign of a zero result), and the reference's score is never -0
+    // (a round-to-nearest sum is -0 only when both addends are, and its
+    // sums start at +0), so the final + 0.0f returns the reference's bits.
+    float S = 0.0f, r;
     if constexpr (LDS_ELITES) {
         // broadcast LDS reads of this lane run; the address is re-opaqued
         // per run so that only one run's weights are live at a time
@@ -952,14 +979,15 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const float acc = w[0] + w[lds_model_run(j)] * g[j];
-            float sum = 0.0f;
+            float sum = w[lds_model_run(j) + 1] * (g[j] * g[j]);
 #pragma unroll
-            for (int i = j; i < 16; ++i) sum = sum + w[lds_model_run(j) + 1 + (i - j)] * (g[j] * g[i]);
+            for (int i = j + 1; i < 16; ++i) sum = sum + w[lds_model_run(j) + 1 + (i - j)] * (g[j] * g[i]);
             const float ln = acc + sum;
             S2 = (j == 0) ? ln : S2 + ln;
         }
+        r = w[153] * g[16];
 #pragma unroll
-        for (int i = 16; i < CS_NTOT; ++i) r = r + w[153 + (i - 16)] * g[i];
+        for (int i = 17; i < CS_NTOT; ++i) r = r + w[153 + (i - 16)] * g[i];
         S = S2;
     } else {
     uint64_t mp = (uint64_t)a.model;
@@ -968,16 +996,17 @@ __device__ __forceinline__ void score_batch(const ScoreArgs &a, const MlpLds &ml
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const float acc = md->w0 + md->lin[j] * g[j];
-        float sum = 0.0f;
+        float sum = md->tri[j * (j + 1) / 2 + j] * (g[j] * g[j]);
 #pragma unroll
-        for (int i = j; i < 16; ++i) sum = sum + md->tri[i * (i + 1) / 2 + j] * (g[j] * g[i]);
+        for (int i = j + 1; i < 16; ++i) sum = sum + md->tri[i * (i + 1) / 2 + j] * (g[j] * g[i]);
         const float ln = acc + sum;
         S = (j == 0) ? ln : S + ln;
     }
+    r = md->lin[16] * g[16];
 #pragma unroll
-    for (int i = 16; i < CS_NTOT; ++i) r = r + md->lin[i] * g[i];
+    for (int i = 17; i < CS_NTOT; ++i) r = r + md->lin[i] * g[i];
     }
-    score = S + r;
+    score = (S + r) + 0.0f;
     }
     if constexpr (SC != SC_MLP_F64) s64 = (double)score;
     const bool pass = act && (SC == SC_MLP_F64 ? !(s64 <= bound64) : !(score <= bound));
@@ -1034,7 +1063,10 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
     constexpr bool LDS_MODEL = LDS_ELITES && SC == SC_QUAD;
     float *mdl_lds = reinterpret_cast<float *>(smem + NW * 2 * QCAP);
     float *el_lds = mdl_lds + (LDS_MODEL ? LDS_MODEL_BYTES / 4 : 0);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // readfirstlane: the wave index is uniform, and saying so keeps the run
+    // bookkeeping derived from it (block, stream, chain position, ring
+    // tails) in scalar registers instead of on the VALU
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const float *el = a.elites;
     const float *mdl = mdl_lds;
     if (LDS_ELITES) {
@@ -1118,13 +1150,16 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
     }
     uint32_t *qN = qbase + wave * 2 * QCAP;
     uint32_t *qI = qN + QCAP;
-    unsigned long long screened = 0;
+    uint32_t screened = 0;  // at most 256 pairs of blocks per wave (grid <= 2^20, nflat < 2^31)
     // Each wave takes pairs of flat blocks; two blocks of one stream are
     // walked as one run (the chain crosses the block boundary by itself), so
     // the partly filled batches at the end of a run come once per two blocks.
-    for (uint64_t fp = (uint64_t)blockIdx.x * NW + wave; 2 * fp < a.nflat; fp += (uint64_t)gridDim.x * NW) {
-    const uint64_t fend = min(2 * fp + 2, a.nflat);
-    for (uint64_t fb = 2 * fp, nb = 1; fb < fend; fb += nb) {
+    // The host keeps nflat below 2^31, so flat block indices are 32-bit
+    // (fewer scalar registers to hold across the scoring loop).
+    const uint32_t nflat = (uint32_t)a.nflat;
+    for (uint32_t fp = blockIdx.x * NW + wave; 2 * fp < nflat; fp += gridDim.x * NW) {
+    const uint32_t fend = min(2 * fp + 2, nflat);
+    for (uint32_t fb = 2 * fp, nb = 1; fb < fend; fb += nb) {
         const int t = find_seg(a.seg_first, a.T, fb);
         nb = (fb + 1 < fend && fb + 1 < a.seg_first[t + 1]) ? 2 : 1;
         const LaunchSeg sg = a.segs[t];
@@ -1133,46 +1168,57 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
         const uint64_t ent = a.entries[st.blk_first + b];
         const uint64_t first = ent >> 8;
         if (first >= st.my_lambda) continue;
-        const uint64_t bstart = b << BLOCK_LOG, bend = bstart + nb * BLOCK;
-        uint64_t c = bstart + (ent & 255u);
-        uint64_t idx = first;
+        const uint64_t bstart = b << BLOCK_LOG;
+        // The chain position p and the candidate count i are kept relative
+        // to the run's start (bstart, first): both fit 32 bits (a run spans
+        // at most two blocks), so the loop's tests are scalar 32-bit
+        // compares.  iend caps i at the stream's last candidate.
+        const uint32_t pend = nb * BLOCK;
+        const uint64_t lam_run = st.my_lambda - first;  // > 0
+        const uint32_t iend = __builtin_amdgcn_readfirstlane((lam_run >> 32) ? 0xffffffffu : (uint32_t)lam_run);
+        uint32_t p = (uint32_t)(ent & 255u), i = 0;
+        const uint64_t seed3 = st.seed + (bstart + 3) * GAMMA;
         uint32_t headN = 0, tailN = 0, headI = 0, tailI = 0;
-        const uint64_t lt = (1ull << lane) - 1ull;
         uint64_t lane_step = (uint64_t)(62u * (uint32_t)lane) * GAMMA;
-        asm volatile("" : "+v"(lane_step));  // opaque: keeps (c + 3) gamma + lane_step from being refactored
+        asm volatile("" : "+v"(lane_step));  // opaque: keeps p gamma + lane_step from being refactored
         // one step of the chain walk: up to 64 candidate starts into the rings
         auto enqueue = [&]() {
-            // speculate: lane j starts at c + 62 j (no immigrant, no jump
-            // before it).  c, idx and n are the same in every lane (scalar
-            // registers); lane j's counter for draw c + 62 j + 3 is
-            // (seed + (c + 3) gamma) + 62 j gamma, one 64-bit add per lane.
-            const uint64_t x3 = st.seed + (c + 3) * GAMMA + lane_step;
+            // speculate: lane j starts at c + 62 j, c = bstart + p (no
+            // immigrant, no jump before it).  p, i and n are the same in
+            // every lane (scalar registers); lane j's counter for draw
+            // c + 62 j + 3 is (seed + (bstart + 3) gamma) + p gamma + 62 j
+            // gamma, one 64-bit multiply-add per lane.
+            const uint64_t x3 = seed3 + (uint64_t)p * GAMMA + lane_step;
             const bool im = draw_u24(x3) < a.immT;
             // the jump test: draw c + 62 j + 33 (immigrant) or + 62
             const bool jp = draw_u24(x3 + (im ? 30ull * GAMMA : 59ull * GAMMA)) < a.jumpT;
             const uint32_t L = im ? (jp ? 35u : 33u) : (jp ? 64u : 62u);
-            const uint64_t stop = __ballot(im || jp);
+            // the masks are formed on the scalar unit from the two ballots
+            const uint64_t bim = __ballot(im), stop = bim | __ballot(jp);
             uint32_t n = stop ? (uint32_t)__ffsll((unsigned long long)stop) : 64u;
-            const uint32_t room = (uint32_t)(bend - c + 61) / 62u;  // c < bend <= c + 2 BLOCK
-            const uint64_t lam = st.my_lambda - idx;
-            n = (uint32_t)min((uint64_t)min(n, room), lam);
+            const uint32_t room = (pend - p + 61) / 62u;  // p < pend
+            // readfirstlane (free on a scalar value) keeps these two scalar
+            // mins from being fused into a VALU min3
+            n = __builtin_amdgcn_readfirstlane(min(n, room));
+            n = min(n, iend - i);
             const bool take = (uint32_t)lane < n;
-            const uint32_t code = ((uint32_t)(c - bstart) + 62u * (uint32_t)lane) |
-                                  (((uint32_t)(idx - first) + (uint32_t)lane) << RUN_POS_BITS) |
+            const uint32_t code = (p + 62u * (uint32_t)lane) | ((i + (uint32_t)lane) << RUN_POS_BITS) |
                                   ((uint32_t)im << 30) | ((uint32_t)jp << 31);
-            const uint64_t mN = __ballot(take && !im), mI = __ballot(take && im);
-            if (take && !im) qN[(tailN + (uint32_t)__popcll(mN & lt)) & (QCAP - 1)] = code;
-            if (take && im) qI[(tailI + (uint32_t)__popcll(mI & lt)) & (QCAP - 1)] = code;
+            const uint64_t mtake = n >= 64 ? ~0ull : (1ull << n) - 1ull;
+            const uint64_t mN = mtake & ~bim, mI = mtake & bim;
+            // ring slot: the tail plus the mask's bits below this lane (mbcnt adds the tail)
+            if (take && !im) qN[below(mN, tailN) & (QCAP - 1)] = code;
+            if (take && im) qI[below(mI, tailI) & (QCAP - 1)] = code;
             tailN += (uint32_t)__popcll(mN);
             tailI += (uint32_t)__popcll(mI);
-            c += 62ull * (n - 1) + (uint64_t)__builtin_amdgcn_readlane(L, (int)(n - 1));
-            idx += n;
+            p += 62u * (n - 1) + (uint32_t)__builtin_amdgcn_readlane(L, (int)(n - 1));
+            i += n;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         };
         if constexpr (SC == SC_QUAD) {
-            while (c < bend && idx < st.my_lambda) {
+            while (p < pend && i < iend) {
                 enqueue();
                 while (tailN - headN >= 64) {
                     score_batch<0, LDS_ELITES, FAST_MUT, SC, DUMP>(a, ml, mdl, el, qN, headN, 64, bstart, first, st.seed,
@@ -1202,11 +1248,11 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
             // One call site for every batch (full or final, either kind):
             // the rings are refilled only when both hold fewer than 64, so
             // they never exceed 127 < QCAP entries.
-            bool more = c < bend && idx < st.my_lambda;
+            bool more = p < pend && i < iend;
             for (;;) {
                 if (more && tailN - headN < 64 && tailI - headI < 64) {
                     enqueue();
-                    more = c < bend && idx < st.my_lambda;
+                    more = p < pend && i < iend;
                     continue;
                 }
                 bool im;
@@ -1228,7 +1274,7 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
         __builtin_amdgcn_wave_barrier();
     }
     }
-    if (lane == 0 && screened) atomicAdd(a.cand_count, screened);
+    if (lane == 0 && screened) atomicAdd(a.cand_count, (unsigned long long)screened);
 }
 
 // ------------------------------------------------------------------ host
@@ -1353,6 +1399,8 @@ static int launch_ranges(HostState &h, const std::vector<uint64_t> &lo, const st
     }
     const uint64_t nflat = first[T];
     if (nflat == 0) return CS_OK;
+    if (nflat >= (1ull << 31))  // k_score indexes flat blocks with 32 bits
+        return fail(CS_E_CAP, "a scoring launch of 2^31 or more blocks (lambda too large for one call)");
     int rc;
     if ((rc = acquire(h, h.B->segs, sizeof(LaunchSeg) * T * 2))) return rc;
     if ((rc = acquire(h, h.B->seg_first, sizeof(uint64_t) * (T + 1) * 2))) return rc;
@@ -1598,6 +1646,8 @@ static int offsets_by_walk(cs_ctx *ctx, HostState &h, uint64_t nblk, const std::
     if (ctx->walk_seg_log) seg_blocks = 1ull << (ctx->walk_seg_log - BLOCK_LOG);
     else if (resident > (uint64_t)T)
         seg_blocks = std::max(MIN_SEG_BLOCKS, (nblk + resident - T - 1) / (resident - T));
+    // walk_segment tests positions as 32-bit offsets within a segment
+    seg_blocks = std::min(seg_blocks, MAX_SEG_BLOCKS);
     std::vector<uint64_t> seg_first(T + 1, 0);
     for (int t = 0; t < T; ++t) {
         const uint64_t nb = h.si[t].nblocks;

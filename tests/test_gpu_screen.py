@@ -106,6 +106,31 @@ def test_all_ties(dev, oracle, golden):
     _run_both(dev, oracle, None, e, 100000, 300, 6, 4, model_w=w)
 
 
+@pytest.mark.parametrize("elites", ["lds", "global"])
+def test_signed_zero_model_scores(dev, oracle, golden, tmp_path, elites):
+    # every weight -0.0: every product is a signed zero.  The reference's
+    # sums start at +0, so every score is +0; the scorer starts its sums at
+    # the first product and must still return +0 (with fewer than 48 KB of
+    # elites they sit in LDS, with 700 the scorer reads them from global
+    # memory and the model through scalar loads)
+    _, e = _c0(golden)
+    if elites == "global":
+        rng = np.random.default_rng(12)
+        pop = tmp_path / "big.txt"
+        with open(pop, "w") as f:
+            for i in range(700):
+                genes = " ".join(f"{x:.4f}" for x in rng.random(28))
+                f.write(f"{i} 1 2 3 {i % 12} {genes}\n")
+        e = str(pop)
+    w = np.full(153, -0.0, np.float32)
+    got, _ = _run_both(dev, oracle, None, e, 1 << 18, 200, 9, 4, model_w=w)
+    assert not np.any(np.signbit(got["score"]))
+    # intercept -0.0 alone, the other weights nonzero
+    w = np.random.default_rng(13).standard_normal(153).astype(np.float32)
+    w[0] = -0.0
+    _run_both(dev, oracle, None, e, 1 << 18, 200, 10, 4, model_w=w)
+
+
 def test_coarse_scores_many_ties(dev, oracle, golden, tmp_path):
     # weights on a coarse lattice and elites on a coarse lattice with no
     # mutation: many exactly equal scores straddle the cut
