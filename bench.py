@@ -46,16 +46,17 @@ PUBLISHED_CPU = 17.2e6    # reference Zig SIMD screen, 4 threads (docs/SCALE.md:
 # (the profile names the source it was measured on; the line reports whether
 # that source is still the one built)
 PROFILED_SOURCE = "codec-arena_amd/csrc/screen_device.hip"
-PROFILED_SHA16 = "9263b5000c910e19"
-VALU_LANE_INSTR = 1859
+PROFILED_SHA16 = "cc16bee24e59b4c7"
+VALU_LANE_INSTR = 1720
 VALU_PEAK = 256 * 128 * 2.4e9  # lane-instructions/s: 256 CUs x 128 lanes/cycle x 2.4 GHz
-SCORE_TRAFFIC_PER_LAUNCH = 9.567e6
-# Multiply-class and 64-bit-add instructions among the 1859 of one candidate:
+SCORE_TRAFFIC_PER_LAUNCH = 9.563e6
+# Multiply-class and 64-bit instructions among the 1720 of one candidate:
 # counted in the ISA of the production k_score (quadratic scorer, elites and
 # model in LDS, fast mutation), in the straight-line synthesis block of a
 # normal candidate (parent picks, crossover bits, gene reads, mutation draws
-# and mutation: 1,128 VALU instructions; the scoring block adds 511 more).
-VALU_CLASS_PER_CANDIDATE = {"v_mad_u64_u32": 98, "v_mul_lo_u32": 195, "v_mul_hi_u32": 36, "v_lshl_add_u64": 63}
+# and mutation: 1,059 VALU instructions; the scoring block adds 492 more).
+VALU_CLASS_PER_CANDIDATE = {"v_mad_u64_u32": 98, "v_mul_lo_u32": 195, "v_mul_hi_u32": 36, "v_lshl_add_u64": 62,
+                            "v_lshrrev_b64": 62}
 
 
 def valu_model(dev_rates, cand_per_s):

@@ -205,7 +205,7 @@ def write_count(path: str, total: int) -> None:
 
 VALU_CLASSES = ("v_add_u32", "v_mul_lo_u32", "v_mul_hi_u32", "v_mad_u64_u32", "v_mul_f32", "v_lshl_add_u64",
                 "v_alignbit_b32", "v_pk_fma_f16", "v_mfma_f32_16x16x32_bf16", "mfma+4pk_fma_f16",
-                "v_mfma_f32_16x16x128_f8f6f4")
+                "v_mfma_f32_16x16x128_f8f6f4", "v_lshrrev_b64")
 
 
 def valu_rates(device: int = 0, iters: int = 1 << 15) -> dict:

@@ -34,7 +34,8 @@ enum RateOp {
     RATE_MFMA_BF16 = 8,     // v_mfma_f32_16x16x32_bf16
     RATE_MFMA_PK_F16 = 9,   // the same MFMAs with four independent v_pk_fma_f16 after each
     RATE_MFMA_F8 = 10,      // v_mfma_f32_16x16x128_f8f6f4 (E4M3 x E4M3)
-    RATE_NOPS = 11
+    RATE_SHR_U64 = 11,      // v_lshrrev_b64
+    RATE_NOPS = 12
 };
 typedef __bf16 pbf16x8 __attribute__((ext_vector_type(8)));
 typedef float pf32x4 __attribute__((ext_vector_type(4)));
@@ -79,6 +80,7 @@ __global__ __launch_bounds__(256) void k_rate(uint32_t *out, uint64_t *cyc, uint
             }
             if constexpr (OP == RATE_MUL_F32) asm volatile("v_mul_f32 %0, %0, %1" : "+v"(f[i]) : "v"(cf));
             if constexpr (OP == RATE_ADD_U64) asm volatile("v_lshl_add_u64 %0, %0, 0, %1" : "+v"(d[i]) : "v"(c64));
+            if constexpr (OP == RATE_SHR_U64) asm volatile("v_lshrrev_b64 %0, 30, %0" : "+v"(d[i]));
             if constexpr (OP == RATE_ALIGNBIT) asm volatile("v_alignbit_b32 %0, %0, %1, 7" : "+v"(a[i]) : "v"(c));
             if constexpr (OP == RATE_PK_FMA_F16) asm volatile("v_pk_fma_f16 %0, %0, %1, %1" : "+v"(a[i]) : "v"(c));
             if constexpr (OP == RATE_MFMA_BF16 || OP == RATE_MFMA_PK_F16)
@@ -132,7 +134,7 @@ extern "C" int cs_valu_rate_probe(int device, int op, uint32_t iters, double *ns
     }
     typedef void (*Kern)(uint32_t *, uint64_t *, uint32_t);
     static const Kern kerns[RATE_NOPS] = {k_rate<0>, k_rate<1>, k_rate<2>, k_rate<3>, k_rate<4>, k_rate<5>,
-                                          k_rate<6>, k_rate<7>, k_rate<8>, k_rate<9>, k_rate<10>};
+                                          k_rate<6>, k_rate<7>, k_rate<8>, k_rate<9>, k_rate<10>, k_rate<11>};
     int rc = CS_OK;
     float ms = 0.0f;
     // one warm-up launch, then the timed one

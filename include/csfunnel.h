@@ -262,7 +262,7 @@ int cs_write_count(const char *path, uint64_t total);
  * v_add_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32, 3 v_mad_u64_u32, 4 v_mul_f32,
  * 5 v_lshl_add_u64, 6 v_alignbit_b32, 7 v_pk_fma_f16, 8 v_mfma_f32_16x16x32_bf16,
  * 9 the same MFMAs with 32 v_pk_fma_f16 beside them, 10 v_mfma_f32_16x16x128_f8f6f4
- * on E4M3 operands) at 8 waves per SIMD on
+ * on E4M3 operands, 11 v_lshrrev_b64) at 8 waves per SIMD on
  * `device`; returns the time per iteration per SIMD (ns), the s_memtime
  * counts per iteration per SIMD (median over waves) and the SIMD count. */
 int cs_valu_rate_probe(int device, int op, uint32_t iters, double *ns_per_iter, double *cycles_per_iter,

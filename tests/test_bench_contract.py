@@ -1,6 +1,7 @@
 """bench.py's contract pieces that need no GPU: the one-GPU refusal and the
 roofline fractions it derives from the device's measured instruction rates."""
 import os
+import re
 import subprocess
 import sys
 
@@ -14,7 +15,7 @@ import bench  # noqa: E402
 
 def _rates(add_ns=1.25, simds=1024):
     ns = {"v_add_u32": add_ns, "v_mul_lo_u32": 1.5 * add_ns, "v_mul_hi_u32": 1.5 * add_ns,
-          "v_mad_u64_u32": 2.0 * add_ns, "v_lshl_add_u64": 1.6 * add_ns,
+          "v_mad_u64_u32": 2.0 * add_ns, "v_lshl_add_u64": 1.6 * add_ns, "v_lshrrev_b64": 1.6 * add_ns,
           "v_mfma_f32_16x16x32_bf16": 11.75, "mfma+4pk_fma_f16": 10.5, "v_pk_fma_f16": 1.8,
           "v_mfma_f32_16x16x128_f8f6f4": 13.5}
     return {"ns_per_instr": ns, "simds": simds}
@@ -65,5 +66,7 @@ def test_profiled_constants_describe_the_built_source():
     with open(os.path.join(REPO, "profiles", "screen_walk_offsets_2p65e9.md")) as f:
         summary = f.read()
     assert f"sha256 prefix `{bench.PROFILED_SHA16}`" in summary
-    assert f"| `k_score` | SQ_INSTS_VALU | 7.699e10 | {bench.VALU_LANE_INSTR}." in summary
+    row = re.search(r"\| `k_score` \| SQ_INSTS_VALU \| ([0-9.e+]+) \| ([0-9.]+) \|", summary)
+    assert row and round(float(row.group(2))) == bench.VALU_LANE_INSTR
+    assert float(row.group(1)) * 64 / 2.65e9 == pytest.approx(float(row.group(2)), abs=0.1)
     assert f"{bench.SCORE_TRAFFIC_PER_LAUNCH / 1e6:.2f} MB per\nlaunch" in summary
