@@ -46,11 +46,11 @@ PUBLISHED_CPU = 17.2e6    # reference Zig SIMD screen, 4 threads (docs/SCALE.md:
 # (the profile names the source it was measured on; the line reports whether
 # that source is still the one built)
 PROFILED_SOURCE = "codec-arena_amd/csrc/screen_device.hip"
-PROFILED_SHA16 = "cc16bee24e59b4c7"
-VALU_LANE_INSTR = 1720
+PROFILED_SHA16 = "e599b6504bf5dc9e"
+VALU_LANE_INSTR = 1716
 VALU_PEAK = 256 * 128 * 2.4e9  # lane-instructions/s: 256 CUs x 128 lanes/cycle x 2.4 GHz
-SCORE_TRAFFIC_PER_LAUNCH = 9.563e6
-# Multiply-class and 64-bit instructions among the 1720 of one candidate:
+SCORE_TRAFFIC_PER_LAUNCH = 9.560e6
+# Multiply-class and 64-bit instructions among the 1716 of one candidate:
 # counted in the ISA of the production k_score (quadratic scorer, elites and
 # model in LDS, fast mutation), in the straight-line synthesis block of a
 # normal candidate (parent picks, crossover bits, gene reads, mutation draws

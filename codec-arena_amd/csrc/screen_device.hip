@@ -1150,7 +1150,7 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
     }
     uint32_t *qN = qbase + wave * 2 * QCAP;
     uint32_t *qI = qN + QCAP;
-    uint32_t screened = 0;  // at most 256 pairs of blocks per wave (grid <= 2^20, nflat < 2^31)
+    unsigned long long screened = 0;
     // Each wave takes pairs of flat blocks; two blocks of one stream are
     // walked as one run (the chain crosses the block boundary by itself), so
     // the partly filled batches at the end of a run come once per two blocks.
@@ -1274,7 +1274,7 @@ __global__ __launch_bounds__(64 * score_waves<SC>()) void k_score(ScoreArgs a) {
         __builtin_amdgcn_wave_barrier();
     }
     }
-    if (lane == 0 && screened) atomicAdd(a.cand_count, (unsigned long long)screened);
+    if (lane == 0 && screened) atomicAdd(a.cand_count, screened);
 }
 
 // ------------------------------------------------------------------ host
