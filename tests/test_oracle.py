@@ -399,6 +399,73 @@ def test_simd_score_form_equals_scalar(oracle, golden):
             assert a.view(np.uint32) == b.view(np.uint32)
 
 
+def _score_first_product_order(m, g):
+    """The device scorer's float32 operation order: each lane's quadratic sum
+    and the linear tail start at their first product instead of at +0, and
+    the score gets a final + 0.0 (numpy float32 arithmetic: round to nearest,
+    no fused multiply-add)."""
+    f = np.float32
+    w0, lin = f(m.w0), np.array(m.lin[:], np.float32)
+    quad = np.array(m.quad[:], np.float32).reshape(16, 16)
+    S = None
+    for j in range(16):
+        acc = f(w0 + f(lin[j] * g[j]))
+        s = f(quad[j, j] * f(g[j] * g[j]))
+        for i in range(j + 1, 16):
+            s = f(s + f(quad[i, j] * f(g[j] * g[i])))
+        ln = f(acc + s)
+        S = ln if j == 0 else f(S + ln)
+    r = f(lin[16] * g[16])
+    for i in range(17, 29):
+        r = f(r + f(lin[i] * g[i]))
+    return f(f(S + r) + f(0.0))
+
+
+def test_score_from_first_product_equals_reference_order(oracle):
+    # The scorer drops the reference's +0 starts of its sums (17 adds per
+    # candidate) and adds +0 once at the end.  The two orders may differ only
+    # in the sign of a zero partial result, and the reference's score is
+    # never -0, so the scores must agree bit for bit.  Inputs full of signed
+    # zeros (weights, genes, the intercept) are where a difference would show.
+    rng = np.random.default_rng(29)
+    palette = np.array([0.0, -0.0, 1.0, -1.0, 0.5, -2.0, 2.0 ** -149, -(2.0 ** -149)], np.float32)
+    zero_scores = 0
+    for case in range(400):
+        n = (153, 166)[(case // 8) % 2]
+        if case % 4 == 3:  # signed zeros only: every score is a zero
+            w = palette[rng.integers(0, 2, n)]
+        elif case % 2:
+            w = palette[rng.integers(0, palette.size, n)]
+        else:
+            w = (rng.standard_normal(n) * (rng.random(n) < 0.3)).astype(np.float32)
+            w[rng.random(n) < 0.3] = -0.0
+        w[0] = [0.0, -0.0, 1.0, [0.0, -0.0][(case // 4) % 2]][case % 4]
+        m = oracle.model_from_compact(w.astype(np.float32))
+        for _ in range(8):
+            g = rng.random(29).astype(np.float32)
+            g[rng.random(29) < 0.5] = 0.0
+            g[rng.random(29) < 0.3] = -0.0
+            want = np.float32(oracle.score(m, g))
+            got = _score_first_product_order(m, g)
+            assert got.view(np.uint32) == want.view(np.uint32), (case, want, got)
+            if want == 0:
+                zero_scores += 1
+                assert not np.signbit(want)
+    assert zero_scores >= 800  # the signed-zero cases were reached
+    # every weight -0.0 and genes >= +0: every product is -0.0, so sums that
+    # start at their first product are -0.0 and only the final + 0.0 gives
+    # the reference's +0
+    # (width 166: the linear tail over genes 16..28 comes from the weights
+    # too; at width 153 it is +0 and would restore the sign by itself)
+    m = oracle.model_from_compact(np.full(166, -0.0, np.float32))
+    for _ in range(20):
+        g = rng.random(29).astype(np.float32)
+        g[rng.random(29) < 0.3] = 0.0
+        want = np.float32(oracle.score(m, g))
+        assert want == 0 and not np.signbit(want)
+        assert _score_first_product_order(m, g).view(np.uint32) == want.view(np.uint32)
+
+
 @pytest.mark.parametrize("lam,T,sigma,jump,imm", [(1 << 20, 4, 0.12, 0.01, 0.01), (1 << 18, 1, 0.12, 0.01, 0.01),
                                                  (200000, 3, 0.5, 0.5, 0.5), (100000, 4, 0.0, 0.0, 0.0)])
 def test_simd_screen_equals_scalar_screen(oracle, golden, lam, T, sigma, jump, imm):
