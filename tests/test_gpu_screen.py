@@ -122,7 +122,9 @@ def test_signed_zero_model_scores(dev, oracle, golden, tmp_path, elites):
                 genes = " ".join(f"{x:.4f}" for x in rng.random(28))
                 f.write(f"{i} 1 2 3 {i % 12} {genes}\n")
         e = str(pop)
-    w = np.full(153, -0.0, np.float32)
+    # width 166: the linear tail over genes 16..28 is -0.0 too (at width 153
+    # it is +0, which alone would restore the sign of a zero score)
+    w = np.full(166, -0.0, np.float32)
     got, _ = _run_both(dev, oracle, None, e, 1 << 18, 200, 9, 4, model_w=w)
     assert not np.any(np.signbit(got["score"]))
     # intercept -0.0 alone, the other weights nonzero
