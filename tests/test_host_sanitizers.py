@@ -1,0 +1,52 @@
+"""Host code under AddressSanitizer and UndefinedBehaviorSanitizer: the CPU
+oracle and the codec label source are compiled with gcc (host only, no GPU
+code) and exercised by the drivers in tests/sanitize/; any sanitizer report
+fails the test."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SAN = ["-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-fno-omit-frame-pointer",
+       "-ffp-contract=off", "-fno-fast-math", "-march=x86-64-v3", "-std=gnu11"]
+
+
+def _build(tmp_path, name, sources, include):
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    exe = str(tmp_path / name)
+    r = subprocess.run(["gcc", *SAN, "-I", include, "-o", exe, *sources, "-lpthread", "-lm"], capture_output=True,
+                       text=True, timeout=300)
+    if r.returncode != 0:
+        if "asan" in r.stderr.lower() or "ubsan" in r.stderr.lower():
+            pytest.skip("sanitizer runtime not available: " + r.stderr[-200:])
+        raise AssertionError(r.stderr[-2000:])
+    return exe
+
+
+def _run(cmd):
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600, env=env)
+    out = r.stdout + r.stderr
+    assert "runtime error" not in out and "AddressSanitizer" not in out, out[-4000:]
+    assert r.returncode == 0, out[-4000:]
+    return r.stdout
+
+
+def test_oracle_under_sanitizers(tmp_path, golden):
+    exe = _build(tmp_path, "oracle_drv", [os.path.join(REPO, "tests", "sanitize", "oracle_drv.c"),
+                                         os.path.join(REPO, "oracle", "screen_oracle.c")],
+                 os.path.join(REPO, "oracle"))
+    out = _run([exe, os.path.join(golden, "c0_model.txt"), os.path.join(golden, "c0_elites.txt")])
+    assert out.strip().endswith("ok")
+
+
+def test_codec_label_source_under_sanitizers(tmp_path):
+    exe = _build(tmp_path, "codec_drv", [os.path.join(REPO, "tests", "sanitize", "codec_drv.c"),
+                                        os.path.join(REPO, "codec-arena_amd", "csrc", "codec_eval.c")],
+                 os.path.join(REPO, "include"))
+    out = _run([exe, "1200"])
+    ok = int(out.split()[1])
+    assert ok > 300  # most clips encode and decode
