@@ -13,11 +13,14 @@ SAN = ["-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=undef
        "-ffp-contract=off", "-fno-fast-math", "-march=x86-64-v3", "-std=gnu11"]
 
 
-def _build(tmp_path, name, sources, include):
-    if shutil.which("gcc") is None:
-        pytest.skip("gcc not available")
+def _build(tmp_path, name, sources, include, cxx=False):
+    cc = "g++" if cxx else "gcc"
+    if shutil.which(cc) is None:
+        pytest.skip(cc + " not available")
     exe = str(tmp_path / name)
-    r = subprocess.run(["gcc", *SAN, "-I", include, "-o", exe, *sources, "-lpthread", "-lm"], capture_output=True,
+    flags = [x if x != "-std=gnu11" or not cxx else "-std=c++17" for x in SAN]
+    inc = [x for d in ([include] if isinstance(include, str) else include) for x in ("-I", d)]
+    r = subprocess.run([cc, *flags, *inc, "-o", exe, *sources, "-lpthread", "-lm"], capture_output=True,
                        text=True, timeout=300)
     if r.returncode != 0:
         if "asan" in r.stderr.lower() or "ubsan" in r.stderr.lower():
@@ -50,3 +53,14 @@ def test_codec_label_source_under_sanitizers(tmp_path):
     out = _run([exe, "1200"])
     ok = int(out.split()[1])
     assert ok > 300  # most clips encode and decode
+
+
+def test_host_abi_code_under_sanitizers(tmp_path, golden):
+    # funnel_host.cpp is host C++ only (no HIP): it builds on its own here
+    exe = _build(tmp_path, "host_abi_drv", [os.path.join(REPO, "tests", "sanitize", "host_abi_drv.cpp"),
+                                           os.path.join(REPO, "codec-arena_amd", "csrc", "funnel_host.cpp")],
+                 [os.path.join(REPO, "include"), os.path.join(REPO, "codec-arena_amd", "csrc")], cxx=True)
+    out = _run([exe, os.path.join(golden, "c0_model.txt"), os.path.join(golden, "c0_elites.txt"), str(tmp_path)])
+    assert out.strip().endswith("ok")
+    assert (tmp_path / "c.txt").read_text() == "2650000000\n"
+    assert len((tmp_path / "p.txt").read_text().splitlines()) == 40
