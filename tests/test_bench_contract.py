@@ -70,3 +70,25 @@ def test_profiled_constants_describe_the_built_source():
     assert row and round(float(row.group(2))) == bench.VALU_LANE_INSTR
     assert float(row.group(1)) * 64 / 2.65e9 == pytest.approx(float(row.group(2)), abs=0.1)
     assert f"{bench.SCORE_TRAFFIC_PER_LAUNCH / 1e6:.2f} MB per\nlaunch" in summary
+
+
+def test_next_elites_fills_every_field():
+    # the chained generations' elites: ids, generation, lineage, tribe, the
+    # 23 genes and 6 regulators of each survivor, in cs_genome's layout
+    sys.path.insert(0, os.path.join(REPO, "codec-arena_amd"))
+    import numpy as np
+    from csfunnel import screen as scr
+    rng = np.random.default_rng(4)
+    surv = np.zeros(128, dtype=scr.SURVIVOR_DTYPE)
+    surv["g"] = rng.random((128, 29)).astype(np.float32)
+    surv["pa"] = rng.integers(0, 2 ** 63, 128)
+    surv["pb"] = rng.integers(0, 2 ** 63, 128)
+    surv["tribe"] = rng.integers(0, 12, 128)
+    el = bench.next_elites(surv, 6)
+    assert len(el) == 128
+    for r in (0, 1, 77, 127):
+        e = el[r]
+        assert (e.id, e.gen, e.pa, e.pb, e.tribe) == (6 * 128 + r, 7, int(surv["pa"][r]), int(surv["pb"][r]),
+                                                      int(surv["tribe"][r]))
+        assert list(e.g) == [float(x) for x in surv["g"][r, :23]]
+        assert list(e.reg) == [float(x) for x in surv["g"][r, 23:]]
